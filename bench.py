@@ -1,0 +1,158 @@
+"""Benchmark: Kripke source iteration on one MI355X.
+
+A step is one source iteration (SteadyStateSolver.cpp:44-104) of the
+on-device path: LTimes, scattering, source, LPlusTimes, the zone-set
+pipelined sweep and the population, over the workload
+
+    64^3 zones (8 zone sets of 32^3), 128 groups, 128 directions (Gauss-
+    Legendre 8 polar x 16 azimuthal, 8 direction sets), P4 scattering
+    (25 moments), FP64, DGZ nesting, seeded 3-material box geometry.
+
+Prints one JSON line.  `value` is the reference figure of merit
+(src/kripke.cpp:495-505): unknowns (D*G*Z) per second per iteration, with all
+fields resident in HBM when the timed region starts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "sweep unknowns/s FP64 (64^3z x128g x128d), % HBM peak; adjoint/forward cost"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+WORKLOAD = dict(zones=(64, 64, 64), groups=128, quad=(8, 16), legendre=4, dset=8, gset=1, zset=(2, 2, 2),
+                geometry=1, seed=20261202)
+# bounded sample of the same workload for the CPU oracle (about 10-30 s of CPU work)
+CPU_SAMPLE = dict(zones=(32, 32, 32), groups=64, quad=(8, 16), legendre=4, dset=8, gset=1, zset=(2, 2, 2),
+                  geometry=1, seed=20261202)
+
+
+def algorithmic_bytes(nd_total, ng, nz, moments, num_sdoms_per_zoneset):
+    """Compulsory HBM bytes per iteration of the metric path at the kept
+    entry-point boundary: LTimes reads psi (8N) and writes phi (8P),
+    LPlusTimes reads phi_out (8P) and writes rhs (8N), the sweep reads rhs
+    and writes psi (16N) and reads sigma_t once per subdomain (8GZ each)."""
+    N = nd_total * ng * nz
+    P = moments * ng * nz
+    return {"ltimes": 8 * N + 8 * P, "lplustimes": 8 * P + 8 * N,
+            "sweep": 16 * N + 8 * ng * nz * num_sdoms_per_zoneset, "N": N, "P": P}
+
+
+def run_gpu(args, layout):
+    import kripke_amd as ka
+    ctx = ka.Context(0)
+    t0 = time.time()
+    hp = ka.HostProblem(ka.input_vars(layout=layout, **WORKLOAD), ctx)
+    setup_s = time.time() - t0
+    k = hp.kernels()
+    hp.init_psi()
+    for _ in range(args.warmup):
+        hp.iterate(1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hp.iterate(1)  # ends with the population read-back (a host sync per iteration)
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    # breakdown pass with hipEvent timers, outside the timed region
+    k.set_timing(True)
+    k.timer_reset()
+    nb = 2
+    for _ in range(nb):
+        hp.iterate(1)
+    names = ["LTimes", "Scattering", "Source", "LPlusTimes", "SweepSolver", "k_sweep", "Population"]
+    timers = {n: k.timer(n) for n in names}
+    k.set_timing(False)
+    hp.close()
+    ctx.close()
+    return elapsed, setup_s, timers, nb
+
+
+def run_cpu_baseline():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    o = po.Oracle(po.make_input(layout="DGZ", **CPU_SAMPLE), kind="ref", omp=True)
+    o.init_psi()
+    iters = 2
+    o.timer_reset()
+    for _ in range(iters):
+        o.iterate()
+    solve, _ = o.timer("Solve")
+    nz = CPU_SAMPLE["zones"][0] * CPU_SAMPLE["zones"][1] * CPU_SAMPLE["zones"][2]
+    N = nz * CPU_SAMPLE["groups"] * CPU_SAMPLE["quad"][0] * CPU_SAMPLE["quad"][1]
+    path = sum(o.timer(n)[0] for n in ("LTimes", "LPlusTimes", "SweepSolver"))
+    return {"value": N * iters / solve, "unit": "unknowns/s", "cores": po.omp_threads(), "kind": "port",
+            "sample": "CPU oracle (OpenMP restatement of the reference OpenMP loop orders), 32^3 zones x 64 groups "
+                      "x 128 directions, P4, 8 zone sets, DGZ, %d iterations; whole-Solve FOM" % iters,
+            "metric_path_unknowns_per_s": N * iters / path, "solve_s": solve}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--layout", default="DGZ")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs as a single process on one GPU", file=sys.stderr)
+        sys.exit(2)
+
+    elapsed, setup_s, timers, nb = run_gpu(args, args.layout)
+    nz = 64 ** 3
+    nd, ng, M = 128, 128, 25
+    ab = algorithmic_bytes(nd, ng, nz, M, num_sdoms_per_zoneset=8)
+    N = ab["N"]
+    ms = elapsed / args.steps * 1e3
+    value = N * args.steps / elapsed
+
+    sweep_t, sweep_n = timers["k_sweep"]
+    lt_t, _ = timers["LTimes"]
+    lp_t, _ = timers["LPlusTimes"]
+    ss_t, _ = timers["SweepSolver"]
+    # sweep kernel roofline: algorithmic bytes of all stage launches / their time
+    sweep_achieved = (ab["sweep"] * nb) / sweep_t if sweep_t > 0 else None
+    path_t = (lt_t + lp_t + ss_t) / nb
+    path_bytes = ab["ltimes"] + ab["lplustimes"] + ab["sweep"]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "unknowns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 3-material boxes in a 100 cm cube, vacuum boundaries)",
+        "config": {"workload": "64^3 zones (2x2x2 zone sets), 128 groups, 128 directions GL 8x16 (8 direction sets), "
+                               "P4 (25 moments), %s nesting, on-device source iteration" % args.layout,
+                   "unknowns": N, "nesting": args.layout, "zone_sets": [2, 2, 2], "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": sweep_achieved / 1e9 if sweep_achieved else None,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": sweep_achieved / HBM_PEAK if sweep_achieved else None, "traffic": None,
+                     "kernel": "k_sweep", "bytes_per_iteration": ab["sweep"],
+                     "avg_launch_ms": sweep_t / max(sweep_n, 1) * 1e3},
+        "metric_path": {"ms_per_iteration": path_t * 1e3, "alg_bytes_per_iteration": path_bytes,
+                        "hbm_frac": path_bytes / path_t / HBM_PEAK if path_t > 0 else None,
+                        "unknowns_per_s": N / path_t if path_t > 0 else None},
+        "breakdown_ms": {n: timers[n][0] / nb * 1e3 for n in timers},
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = run_cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,204 @@
+/*
+ * kripke_amd.h — C-ABI of the MI355X (gfx950) source-iteration hot path.
+ *
+ * Drop-in boundary for LLNL/Kripke's Kripke::Kernel entry points
+ * (src/Kripke/Kernel.h:19-34) and its sweep driver
+ * (src/Kripke/SweepSolver.cpp:23-87).  The reference's host code keeps its
+ * DataStore/Set/Field objects; each Kernel:: function looks its fields up by
+ * name as it does today and hands raw device pointers, extents and the
+ * nesting to the functions below.  Only plain C types cross this boundary.
+ *
+ * Conventions
+ *  - Every function returns KR_OK (0) or a KR_ERR_* code; kr_last_error()
+ *    gives a message for the calling thread.  Nothing throws across it.
+ *  - All work is ordered on the context's HIP stream and is asynchronous,
+ *    except kr_population (returns a scalar) and the explicit copies.
+ *  - Field chunks are laid out in the DefaultOrder of the nesting
+ *    (src/Kripke/VarTypes.h:68-101): the index type listed first has the
+ *    longest stride.  Flat offsets are 64-bit.
+ *  - A context and the problems on it are not thread-safe.
+ */
+#ifndef KRIPKE_AMD_H
+#define KRIPKE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KR_OK 0
+#define KR_ERR_INVALID 1     /* bad argument or descriptor */
+#define KR_ERR_HIP 2         /* HIP runtime error */
+#define KR_ERR_NOMEM 3       /* device allocation failed */
+#define KR_ERR_UNSUPPORTED 4 /* shape outside what the kernels were built for */
+#define KR_ERR_NO_DEVICE 5   /* no HIP device available */
+
+/* LayoutV (src/Kripke/ArchLayout.h:92-101), same numbering. */
+typedef enum kr_nesting { KR_DGZ = 0, KR_DZG = 1, KR_GDZ = 2, KR_GZD = 3, KR_ZDG = 4, KR_ZGD = 5 } kr_nesting;
+
+typedef struct kr_context kr_context;
+typedef struct kr_problem kr_problem;
+
+/* One entry per subdomain (SdomId), in SdomId order.  Pointers are device
+ * pointers to that subdomain's chunk of the named reference field. */
+typedef struct kr_sdom_desc {
+  int32_t gset, dset, zx, zy, zz;    /* PartitionSpace coordinates (PartitionSpace.cpp:97) */
+  int32_t pr_chunk, r_chunk;         /* chunk ids in SPACE_PR and SPACE_R */
+  int32_t upwind[3], downwind[3];    /* "upwind"/"downwind" (Field_Adjacency), -1 = boundary */
+  int32_t num_directions, num_groups;/* Set/Direction, Set/Group sizes */
+  int32_t ni, nj, nk;                /* Set/ZoneI, ZoneJ, ZoneK sizes */
+  int32_t direction_lower, group_lower; /* Set::lower of Direction and Group */
+  int32_t reserved;
+  double *psi, *rhs;                 /* "psi", "rhs" (Field_Flux) */
+  double *i_plane, *j_plane, *k_plane; /* "i_plane", "j_plane", "k_plane" */
+  const double *xcos, *ycos, *zcos, *w; /* "quadrature/{xcos,ycos,zcos,w}" */
+  const int32_t *id, *jd, *kd;       /* "quadrature/{id,jd,kd}" */
+  const double *ell, *ell_plus;      /* "ell" (Field_Ell), "ell_plus" (Field_EllPlus) */
+  const double *dx, *dy, *dz;        /* "dx", "dy", "dz" */
+} kr_sdom_desc;
+
+/* One entry per SPACE_PR chunk (group set x zone set). */
+typedef struct kr_pr_desc {
+  double *phi, *phi_out;             /* "phi", "phi_out" (Field_Moments) */
+  const double *sigt_zonal;          /* "sigt_zonal" (Field_SigmaTZonal) */
+} kr_pr_desc;
+
+/* One entry per SPACE_R chunk (zone set). */
+typedef struct kr_r_desc {
+  const double *volume;              /* "volume" (Field_Zone2Double) */
+  const int64_t *zone_to_mixelem;    /* "zone_to_mixelem" (Field_Zone2MixElem) */
+  const int32_t *zone_to_num_mixelem;/* "zone_to_num_mixelem" (Field_Zone2Int) */
+  const int64_t *mixelem_to_zone;    /* "mixelem_to_zone" */
+  const int64_t *mixelem_to_material;/* "mixelem_to_material" */
+  const double *mixelem_to_fraction; /* "mixelem_to_fraction" */
+  int64_t num_mixelem;               /* Set/MixElem size */
+} kr_r_desc;
+
+typedef struct kr_problem_desc {
+  int32_t nesting;                   /* kr_nesting */
+  int32_t num_sdoms, num_pr, num_r;
+  int32_t num_moments;               /* Set/Moment size, (L+1)^2 */
+  int32_t num_legendre;              /* Set/Legendre size, L+1 */
+  int32_t num_materials;             /* Set/Material size */
+  int32_t global_num_groups;         /* Set/GlobalGroup size */
+  const kr_sdom_desc *sdoms;         /* host array, num_sdoms entries */
+  const kr_pr_desc *pr;              /* host array, num_pr entries */
+  const kr_r_desc *r;                /* host array, num_r entries */
+  const double *sigs;                /* "data/sigs" (Field_SigmaS), device */
+  const int64_t *moment_to_legendre; /* "moment_to_legendre", device */
+} kr_problem_desc;
+
+/* ---- context and device memory ---------------------------------------- */
+int kr_context_create(int32_t device, kr_context **out);
+int kr_context_destroy(kr_context *ctx);
+const char *kr_last_error(void);
+int kr_device_count(int32_t *count);
+int kr_malloc(kr_context *ctx, size_t bytes, void **dptr);
+int kr_free(kr_context *ctx, void *dptr);
+/* Copies are ordered on the context stream and complete before returning. */
+int kr_memcpy_htod(kr_context *ctx, void *dst, const void *src, size_t bytes);
+int kr_memcpy_dtoh(kr_context *ctx, void *dst, const void *src, size_t bytes);
+int kr_synchronize(kr_context *ctx);
+
+/* ---- problem registration ---------------------------------------------- */
+/* Copies the descriptor tables (not the field data) and precomputes the
+ * sweep coefficients 2*cos/delta per (direction, zone index) exactly as
+ * SweepSubdomain.cpp:79-81 computes them.  Field pointers must stay valid
+ * until kr_problem_destroy. */
+int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem **out);
+int kr_problem_destroy(kr_problem *prob);
+
+/* ---- Kernel:: entry points --------------------------------------------- */
+/* Kernel::kConst (src/Kripke/Kernel.h:37-55): chunk[i] = value. */
+int kr_kconst(kr_context *ctx, double *chunk, int64_t num_elem, double value);
+/* Element copy dst[i] = src[i]; the on-rank plane hand-off of
+ * ParallelComm::postSends (src/Kripke/ParallelComm.cpp:157-161). */
+int kr_kcopy(kr_context *ctx, double *dst, const double *src, int64_t num_elem);
+
+/* Kernel::LTimes (src/Kripke/Kernel/LTimes.cpp:79-105):
+ * phi(nm,g,z) (+)= sum over subdomains, directions of ell(nm,d)*psi(d,g,z).
+ * accumulate != 0 adds to phi as the reference does; accumulate == 0
+ * starts from +0.0, which equals kConst(phi,0) followed by LTimes. */
+int kr_ltimes(kr_problem *prob, int32_t accumulate);
+/* Kernel::LPlusTimes (src/Kripke/Kernel/LPlusTimes.cpp:68-92):
+ * rhs(d,g,z) (+)= sum_nm ell_plus(d,nm)*phi_out(nm,g,z). */
+int kr_lplustimes(kr_problem *prob, int32_t accumulate);
+/* Kernel::scattering (src/Kripke/Kernel/Scattering.cpp:112-164):
+ * phi_out(nm,g,z) (+)= sum_gp sigs_z(n,g,gp)*phi(nm,gp,z) over subdomain
+ * pairs sharing a zone set. */
+int kr_scattering(kr_problem *prob, int32_t accumulate);
+/* Kernel::source (src/Kripke/Kernel/Source.cpp:83-115):
+ * phi_out(0,g,z) += strength*fraction for material-0 mixed elements. */
+int kr_source(kr_problem *prob, double source_strength);
+/* Kernel::sweepSubdomain (src/Kripke/Kernel/SweepSubdomain.cpp:115-124):
+ * diamond-difference sweep of one subdomain; reads the incoming faces from
+ * its own plane chunks and leaves the outgoing faces there. */
+int kr_sweep_subdomain(kr_problem *prob, int32_t sdom_id);
+/* SweepSolver (src/Kripke/SweepSolver.cpp:23-87) with the on-rank
+ * SweepComm hand-off: sweeps every listed subdomain in dependency order,
+ * vacuum faces read as zero, outgoing faces written straight into the
+ * downwind subdomain's plane chunk.  Upwind subdomains must be listed. */
+int kr_sweep_solver(kr_problem *prob, const int32_t *sdom_list, int32_t num);
+/* Kernel::population (src/Kripke/Kernel/Population.cpp:74-101):
+ * sum of w(d)*psi(d,g,z)*volume(z); deterministic fixed-order tree sum.
+ * Blocks until the value is on the host. */
+int kr_population(kr_problem *prob, double *out);
+
+/* ---- per-entry-point timing (hipEvents on the context stream) ---------- */
+/* Names: LTimes, LPlusTimes, Scattering, Source, SweepSubdomain,
+ * SweepSolver, Population, kConst.  kr_problem_timer synchronizes. */
+int kr_problem_set_timing(kr_problem *prob, int32_t enable);
+int kr_problem_timer(kr_problem *prob, const char *name, double *seconds, int64_t *count);
+int kr_problem_timer_reset(kr_problem *prob);
+
+/* ======================================================================= */
+/* Host mirror: a from-scratch C++ DataStore/Set/Field + generateProblem +  */
+/* SteadyStateSolver that drives the entry points above.                   */
+/* ======================================================================= */
+
+/* InputVariables (src/Kripke/InputVariables.h:19-46) plus the seeded
+ * geometry of the benchmark configurations. */
+typedef struct kr_input_vars {
+  int32_t nx, ny, nz;
+  int32_t num_directions;
+  int32_t quad_num_polar, quad_num_azimuthal; /* 0,0 = dummy S2 set */
+  int32_t num_groups;
+  int32_t legendre_order;
+  int32_t layout;                    /* kr_nesting */
+  int32_t num_dirsets, num_groupsets;
+  int32_t num_zonesets[3];
+  int32_t num_material_subsamples;
+  int32_t geometry;                  /* 0 = Kobayashi-3i (Space.cpp:142-177), 1 = seeded boxes in a 100 cm cube */
+  uint64_t seed;
+  double sigt[3], sigs[3];           /* used by geometry 0 */
+  double source_strength;            /* Source.cpp:98 uses 1.0 */
+} kr_input_vars;
+
+typedef struct kr_host_problem kr_host_problem;
+
+/* generateProblem (src/Kripke/Generate.cpp:24-94).  ctx == NULL builds the
+ * host-side setup fields only (no device memory; solver calls fail). */
+int kr_host_create(kr_context *ctx, const kr_input_vars *in, kr_host_problem **out);
+int kr_host_destroy(kr_host_problem *hp);
+int kr_host_num_sdoms(kr_host_problem *hp, int32_t *num_sdoms, int32_t *num_pr, int32_t *num_r);
+kr_problem *kr_host_kernel_problem(kr_host_problem *hp);
+/* Field access by reference name and chunk (element count returned in
+ * *num_elem; dst may be NULL to query).  Integer fields convert to double. */
+int kr_host_field_get(kr_host_problem *hp, const char *name, int32_t chunk, double *dst, int64_t *num_elem);
+int kr_host_field_set(kr_host_problem *hp, const char *name, int32_t chunk, const double *src);
+/* SteadyStateSolver (src/Kripke/SteadyStateSolver.cpp:24-115).
+ * mode 0: the reference call sequence (kConst fills, accumulating entry
+ *         points, host-side SweepComm queue over kr_sweep_subdomain).
+ * mode 1: the on-device pipeline (fills folded into the kernels,
+ *         kr_sweep_solver).  Both give the same phi and psi.
+ * kr_host_init_psi zeroes psi (SteadyStateSolver.cpp:39). */
+int kr_host_init_psi(kr_host_problem *hp);
+int kr_host_iterate(kr_host_problem *hp, int32_t mode, double *population);
+int kr_host_solve(kr_host_problem *hp, int32_t niter, int32_t mode, double *populations);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KRIPKE_AMD_H */

@@ -1,0 +1,82 @@
+// Host mirror of the reference's hot-path interface: generateProblem, the
+// Kripke::Kernel entry points, SteadyStateSolver and SweepSolver.  Same
+// names and call contracts as src/Kripke/{Generate.h,Kernel.h,
+// SteadyStateSolver.h,SweepSolver.h}; the bodies call the C-ABI.
+#pragma once
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "core.h"
+
+namespace Kripke {
+
+using InputVariables = kr_input_vars;
+
+// "al": ArchLayout (src/Kripke/ArchLayout.h).  One architecture here (HIP).
+struct ArchLayout : public Core::BaseVar {
+  int layout = KR_DGZ;
+};
+
+// Registered kernel problem ("kr_problem" in the DataStore).
+struct KernelProblem : public Core::BaseVar {
+  kr_problem *prob = nullptr;
+  kr_context *ctx = nullptr;
+  ~KernelProblem() override {
+    if (prob) kr_problem_destroy(prob);
+  }
+};
+
+// Material data kept after generation (sigt/sigs per material).
+struct MaterialData : public Core::BaseVar {
+  double sigt[3], sigs[3];
+  double source_strength = 1.0;
+};
+
+// Named accumulating timers with the reference names (src/Kripke/Timing.h).
+struct Timing : public Core::BaseVar {
+  struct T {
+    double total = 0;
+    int64_t count = 0;
+  };
+  std::vector<std::pair<std::string, T>> timers;
+  T &get(const std::string &n) {
+    for (auto &kv : timers)
+      if (kv.first == n) return kv.second;
+    timers.push_back({n, T{}});
+    return timers.back().second;
+  }
+};
+
+void generateProblem(Core::DataStore &ds, const InputVariables &in, kr_context *ctx);
+
+namespace Kernel {
+void LTimes(Core::DataStore &ds);
+void LPlusTimes(Core::DataStore &ds);
+void scattering(Core::DataStore &ds);
+void source(Core::DataStore &ds);
+void sweepSubdomain(Core::DataStore &ds, SdomId sdom_id);
+double population(Core::DataStore &ds);
+// kConst over every chunk of a field / one chunk (src/Kripke/Kernel.h:37-55)
+void kConst(Core::Field<double> &field, double value);
+void kConst(Core::Field<double> &field, SdomId sdom_id, double value);
+// Variants used by the on-device pipeline: start from +0.0 instead of
+// reading a zero-filled field (same results, no fill pass).
+void LTimesOverwrite(Core::DataStore &ds);
+void LPlusTimesOverwrite(Core::DataStore &ds);
+void scatteringOverwrite(Core::DataStore &ds);
+}  // namespace Kernel
+
+// src/Kripke/SweepSolver.cpp:23-87.  pipeline=true uses kr_sweep_solver.
+void SweepSolver(Core::DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline);
+// One iteration of src/Kripke/SteadyStateSolver.cpp:44-104.
+double SteadyStateIteration(Core::DataStore &ds, int mode);
+
+// Thrown on any C-ABI failure inside the host mirror (KRIPKE_ABORT analogue).
+struct KripkeError : public std::runtime_error {
+  int code;
+  KripkeError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+void check(int rc, const char *what);
+
+}  // namespace Kripke

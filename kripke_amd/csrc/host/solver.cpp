@@ -1,0 +1,123 @@
+// Kripke::Kernel entry points, SweepSolver and the steady-state iteration
+// of the host mirror, driving the C-ABI (src/Kripke/Kernel/*.cpp,
+// src/Kripke/SweepSolver.cpp, src/Kripke/SteadyStateSolver.cpp).
+#include <cstring>
+
+#include "kripke_host.h"
+
+using namespace Kripke;
+using namespace Kripke::Core;
+
+namespace {
+kr_problem *kp(DataStore &ds) { return ds.getVariable<KernelProblem>("kr_problem").prob; }
+}  // namespace
+
+void Kernel::kConst(Field<double> &field, SdomId sdom_id, double value) {
+  check(kr_kconst(field.context(), field.device(sdom_id), (int64_t)field.size(sdom_id), value), "kConst");
+}
+
+void Kernel::kConst(Field<double> &field, double value) {
+  for (SdomId s : field.getWorkList()) kConst(field, s, value);
+}
+
+void Kernel::LTimes(DataStore &ds) { check(kr_ltimes(kp(ds), 1), "LTimes"); }
+void Kernel::LPlusTimes(DataStore &ds) { check(kr_lplustimes(kp(ds), 1), "LPlusTimes"); }
+void Kernel::scattering(DataStore &ds) { check(kr_scattering(kp(ds), 1), "scattering"); }
+void Kernel::LTimesOverwrite(DataStore &ds) { check(kr_ltimes(kp(ds), 0), "LTimes"); }
+void Kernel::LPlusTimesOverwrite(DataStore &ds) { check(kr_lplustimes(kp(ds), 0), "LPlusTimes"); }
+void Kernel::scatteringOverwrite(DataStore &ds) { check(kr_scattering(kp(ds), 0), "scattering"); }
+
+void Kernel::source(DataStore &ds) {
+  check(kr_source(kp(ds), ds.getVariable<MaterialData>("materials").source_strength), "source");
+}
+
+void Kernel::sweepSubdomain(DataStore &ds, SdomId sdom_id) {
+  check(kr_sweep_subdomain(kp(ds), sdom_id), "sweepSubdomain");
+}
+
+double Kernel::population(DataStore &ds) {
+  double part = 0.0;
+  check(kr_population(kp(ds), &part), "population");
+  return part;  // Comm::allReduceSumDouble is the identity on one rank
+}
+
+// SweepSolver with the on-rank SweepComm (SweepSolver.cpp:23-87,
+// ParallelComm.cpp:61-263).  pipeline=false replays the reference queue on
+// the host; pipeline=true hands the whole list to the on-device pipeline.
+void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline) {
+  if (pipeline) {
+    check(kr_sweep_solver(kp(ds), sdom_list.data(), (int32_t)sdom_list.size()), "SweepSolver");
+    return;
+  }
+  auto &ipl = ds.getVariable<Field<double>>("i_plane");
+  auto &jpl = ds.getVariable<Field<double>>("j_plane");
+  auto &kpl = ds.getVariable<Field<double>>("k_plane");
+  Field<double> *planes[3] = {&ipl, &jpl, &kpl};
+  Kernel::kConst(ipl, 0.0);
+  Kernel::kConst(jpl, 0.0);
+  Kernel::kConst(kpl, 0.0);
+  auto &up = ds.getVariable<Field<int32_t>>("upwind");
+  auto &down = ds.getVariable<Field<int32_t>>("downwind");
+  std::vector<SdomId> q_ids;
+  std::vector<int> q_dep;
+  for (SdomId s : sdom_list) {  // postRecvs: count on-rank upwind dependencies
+    int n = 0;
+    for (int dim = 0; dim < 3; ++dim)
+      if (up.host(s)[dim] >= 0) ++n;
+    q_ids.push_back(s);
+    q_dep.push_back(n);
+  }
+  while (!q_ids.empty()) {
+    int pick = -1;
+    for (size_t i = 0; i < q_ids.size(); ++i)
+      if (q_dep[i] == 0) {
+        pick = (int)i;
+        break;
+      }
+    if (pick < 0) throw KripkeError(KR_ERR_INVALID, "SweepSolver: no ready subdomain (missing upwind subdomain)");
+    const SdomId s = q_ids[pick];
+    for (int dim = 0; dim < 3; ++dim)
+      if (up.host(s)[dim] == -1) Kernel::kConst(*planes[dim], s, 0.0);
+    Kernel::sweepSubdomain(ds, s);
+    q_ids.erase(q_ids.begin() + pick);
+    q_dep.erase(q_dep.begin() + pick);
+    for (int dim = 0; dim < 3; ++dim) {  // postSends, on-rank branch
+      const int dn = down.host(s)[dim];
+      if (dn < 0) continue;
+      for (size_t i = 0; i < q_ids.size(); ++i)
+        if (q_ids[i] == dn) {
+          q_dep[i]--;
+          break;
+        }
+      check(kr_kcopy(planes[dim]->context(), planes[dim]->device(dn), planes[dim]->device(s),
+                     (int64_t)planes[dim]->size(s)),
+            "plane hand-off");
+    }
+  }
+}
+
+// SteadyStateSolver.cpp:44-104.  mode 0: the reference call sequence;
+// mode 1: fills folded into the kernels and the on-device sweep pipeline.
+double Kripke::SteadyStateIteration(DataStore &ds, int mode) {
+  auto &pspace = ds.getVariable<PartitionSpace>("pspace");
+  const int n = (int)pspace.getNumSubdomains(SPACE_PQR);
+  std::vector<SdomId> all(n);
+  for (int i = 0; i < n; ++i) all[i] = i;
+  if (mode == 0) {
+    Kernel::kConst(ds.getVariable<Field<double>>("phi"), 0.0);
+    Kernel::LTimes(ds);
+    Kernel::kConst(ds.getVariable<Field<double>>("phi_out"), 0.0);
+    Kernel::scattering(ds);
+    Kernel::source(ds);
+    Kernel::kConst(ds.getVariable<Field<double>>("rhs"), 0.0);
+    Kernel::LPlusTimes(ds);
+    SweepSolver(ds, all, false);
+  } else {
+    Kernel::LTimesOverwrite(ds);
+    Kernel::scatteringOverwrite(ds);
+    Kernel::source(ds);
+    Kernel::LPlusTimesOverwrite(ds);
+    SweepSolver(ds, all, true);
+  }
+  return Kernel::population(ds);
+}

@@ -1,0 +1,665 @@
+// C-ABI layer: contexts, device memory, problem registration and the
+// Kernel:: entry points declared in include/kripke_amd.h.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kr_launch.h"
+
+namespace kr {
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace kr
+
+using namespace kr;
+
+struct kr_context {
+  int device;
+  hipStream_t stream;
+};
+
+struct TimedCall {
+  std::string name;
+  hipEvent_t start, stop;
+};
+
+struct kr_problem {
+  kr_context *ctx = nullptr;
+  int nesting = 0, nsdom = 0, npr = 0, nr = 0, M = 0, L1 = 0, nmat = 0, G = 0;
+  std::vector<kr_sdom_desc> sdoms;
+  std::vector<kr_pr_desc> prs;
+  std::vector<kr_r_desc> rs;
+  std::vector<DevSdom> h_sd;
+  std::vector<DevPR> h_pr;
+  std::vector<int32_t> pr_gset, pr_r;
+  DevSdom *d_sd = nullptr;
+  DevPR *d_pr = nullptr;
+  DevR *d_r = nullptr;
+  int32_t *d_pr_sdoms = nullptr;
+  double *d_coef = nullptr;
+  SweepPlanes *d_planes_ref = nullptr, *d_planes_pipe = nullptr;
+  // sweep schedules
+  SweepShape shape_ref{}, shape_pipe{};
+  SweepUnit *d_units_ref = nullptr;            // all sdoms, grouped by sdom
+  std::vector<int32_t> ref_unit_begin, ref_unit_count;
+  SweepUnit *d_units_pipe = nullptr;           // all sdoms, grouped by level
+  std::vector<int32_t> pipe_level_begin, pipe_level_count;
+  SweepUnit *d_units_tmp = nullptr;            // scratch for arbitrary lists
+  int32_t units_tmp_cap = 0;
+  // population
+  void *d_pop_blocks = nullptr;
+  int64_t pop_nblocks = 0;
+  double *d_pop_partial = nullptr, *d_pop_out = nullptr;
+  // scattering
+  const double *sigs = nullptr;
+  const int64_t *m2l = nullptr;
+  int64_t sigs_str[4] = {0, 0, 0, 0};
+  int32_t *d_sigs_flags = nullptr;
+  int64_t max_gz = 0, max_gmix = 0;
+  // timing
+  int timing = 0;
+  std::vector<TimedCall> pending;
+  std::map<std::string, std::pair<double, int64_t>> timers;
+};
+
+namespace {
+
+int fail(int code, const std::string &msg) {
+  set_error(msg);
+  return code;
+}
+
+template <class T>
+int upload(kr_context *ctx, T **dptr, const std::vector<T> &v) {
+  if (v.empty()) {
+    *dptr = nullptr;
+    return KR_OK;
+  }
+  KR_HIP_CHECK(hipMalloc((void **)dptr, v.size() * sizeof(T)));
+  KR_HIP_CHECK(hipMemcpyAsync(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return KR_OK;
+}
+
+template <class T>
+int download(kr_context *ctx, std::vector<T> &v, const T *dptr, size_t n) {
+  v.resize(n);
+  if (n == 0) return KR_OK;
+  KR_HIP_CHECK(hipMemcpyAsync(v.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return KR_OK;
+}
+
+LaunchCtx launch_ctx(kr_problem *p) {
+  LaunchCtx L{};
+  L.stream = p->ctx->stream;
+  L.sd = p->d_sd;
+  L.pr = p->d_pr;
+  L.r = p->d_r;
+  L.pr_sdoms = p->d_pr_sdoms;
+  L.sigs = p->sigs;
+  L.m2l = p->m2l;
+  for (int i = 0; i < 4; ++i) L.sigs_str[i] = p->sigs_str[i];
+  L.sigs_dims[0] = p->nmat;
+  L.sigs_dims[1] = p->L1;
+  L.sigs_dims[2] = p->G;
+  L.sigs_dims[3] = p->G;
+  L.sigs_flags = p->d_sigs_flags;
+  L.num_pr = p->npr;
+  L.num_moments = p->M;
+  L.max_gz = p->max_gz;
+  L.max_gmix = p->max_gmix;
+  return L;
+}
+
+// Scoped event timing of one entry point on the context stream.
+struct CallTimer {
+  kr_problem *p;
+  TimedCall tc;
+  bool on;
+  CallTimer(kr_problem *pp, const char *name) : p(pp), on(pp->timing != 0) {
+    if (!on) return;
+    tc.name = name;
+    hipEventCreate(&tc.start);
+    hipEventCreate(&tc.stop);
+    hipEventRecord(tc.start, p->ctx->stream);
+  }
+  ~CallTimer() {
+    if (!on) return;
+    hipEventRecord(tc.stop, p->ctx->stream);
+    p->pending.push_back(tc);
+  }
+};
+
+int flush_timers(kr_problem *p) {
+  if (p->pending.empty()) return KR_OK;
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  for (auto &tc : p->pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, tc.start, tc.stop);
+    auto &t = p->timers[tc.name];
+    t.first += ms * 1e-3;
+    t.second += 1;
+    hipEventDestroy(tc.start);
+    hipEventDestroy(tc.stop);
+  }
+  p->pending.clear();
+  return KR_OK;
+}
+
+// Units for the pair groups of one subdomain, R pair groups each.
+void append_units(std::vector<SweepUnit> &u, int sdom, int32_t pgs, int32_t R) {
+  for (int32_t b = 0; b < pgs; b += R) u.push_back(SweepUnit{sdom, b, std::min(R, pgs - b), 0});
+}
+
+int32_t pick_R(int64_t total_pgs, const SweepShape &sh) {
+  const int nt = sh.Q * sh.PJ * sh.PK;
+  const int64_t slots = 256LL * std::max(1, 1024 / std::max(nt, 1));  // resident workgroups, one per 1024 threads
+  int64_t R = total_pgs / std::max<int64_t>(slots, 1);
+  if (R < 1) R = 1;
+  if (R > 64) R = 64;
+  return (int32_t)R;
+}
+
+// Dependency levels of a subdomain list from the on-rank adjacency.
+int sweep_levels(const kr_problem *p, const std::vector<int32_t> &list, std::vector<std::vector<int32_t>> &levels) {
+  std::vector<int> level(p->nsdom, -1), inlist(p->nsdom, 0);
+  for (int32_t s : list) {
+    if (s < 0 || s >= p->nsdom) return fail(KR_ERR_INVALID, "sweep: subdomain id out of range");
+    inlist[s] = 1;
+  }
+  for (int32_t s : list)
+    for (int dim = 0; dim < 3; ++dim) {
+      int u = p->sdoms[s].upwind[dim];
+      if (u >= 0 && !inlist[u]) return fail(KR_ERR_INVALID, "sweep: an upwind subdomain is not in the list");
+    }
+  bool changed = true;
+  int guard = 0;
+  while (changed) {
+    changed = false;
+    if (++guard > p->nsdom + 2) return fail(KR_ERR_INVALID, "sweep: dependency cycle");
+    for (int32_t s : list) {
+      int lv = 0;
+      bool ready = true;
+      for (int dim = 0; dim < 3; ++dim) {
+        int u = p->sdoms[s].upwind[dim];
+        if (u < 0) continue;
+        if (level[u] < 0) { ready = false; break; }
+        lv = std::max(lv, level[u] + 1);
+      }
+      if (ready && level[s] != lv) {
+        level[s] = lv;
+        changed = true;
+      }
+    }
+  }
+  int maxl = -1;
+  for (int32_t s : list) {
+    if (level[s] < 0) return fail(KR_ERR_INVALID, "sweep: unresolved dependency");
+    maxl = std::max(maxl, level[s]);
+  }
+  levels.assign(maxl + 1, {});
+  for (int32_t s : list) levels[level[s]].push_back(s);
+  return KR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *kr_last_error(void) { return g_last_error.c_str(); }
+
+int kr_device_count(int32_t *count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return KR_OK;
+}
+
+int kr_context_create(int32_t device, kr_context **out) {
+  if (!out) return fail(KR_ERR_INVALID, "kr_context_create: null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(KR_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(KR_ERR_INVALID, "kr_context_create: bad device index");
+  KR_HIP_CHECK(hipSetDevice(device));
+  kr_context *c = new kr_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(KR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return KR_OK;
+}
+
+int kr_context_destroy(kr_context *ctx) {
+  if (!ctx) return KR_OK;
+  hipStreamSynchronize(ctx->stream);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return KR_OK;
+}
+
+int kr_malloc(kr_context *ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return fail(KR_ERR_INVALID, "kr_malloc: null argument");
+  KR_HIP_CHECK(hipSetDevice(ctx->device));
+  hipError_t e = hipMalloc(dptr, bytes ? bytes : 8);
+  if (e != hipSuccess) return fail(KR_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return KR_OK;
+}
+
+int kr_free(kr_context *ctx, void *dptr) {
+  if (!ctx) return fail(KR_ERR_INVALID, "kr_free: null context");
+  if (!dptr) return KR_OK;
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  KR_HIP_CHECK(hipFree(dptr));
+  return KR_OK;
+}
+
+int kr_memcpy_htod(kr_context *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return fail(KR_ERR_INVALID, "kr_memcpy_htod: null context");
+  if (!bytes) return KR_OK;
+  KR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return KR_OK;
+}
+
+int kr_memcpy_dtoh(kr_context *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return fail(KR_ERR_INVALID, "kr_memcpy_dtoh: null context");
+  if (!bytes) return KR_OK;
+  KR_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return KR_OK;
+}
+
+int kr_synchronize(kr_context *ctx) {
+  if (!ctx) return fail(KR_ERR_INVALID, "kr_synchronize: null context");
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return KR_OK;
+}
+
+int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem **out) {
+  if (!ctx || !desc || !out) return fail(KR_ERR_INVALID, "kr_problem_create: null argument");
+  const kr_problem_desc &D = *desc;
+  if (D.nesting < 0 || D.nesting > 5) return fail(KR_ERR_INVALID, "kr_problem_create: nesting out of range");
+  if (D.num_sdoms <= 0 || D.num_pr <= 0 || D.num_r <= 0 || !D.sdoms || !D.pr || !D.r)
+    return fail(KR_ERR_INVALID, "kr_problem_create: empty subdomain tables");
+  if (D.num_moments <= 0 || D.num_legendre <= 0 || D.num_moments != D.num_legendre * D.num_legendre)
+    return fail(KR_ERR_INVALID, "kr_problem_create: moments must be (L+1)^2");
+  if (!D.sigs || !D.moment_to_legendre || D.num_materials <= 0 || D.global_num_groups <= 0)
+    return fail(KR_ERR_INVALID, "kr_problem_create: missing scattering data");
+  KR_HIP_CHECK(hipSetDevice(ctx->device));
+
+  kr_problem *p = new kr_problem();
+  p->ctx = ctx;
+  p->nesting = D.nesting;
+  p->nsdom = D.num_sdoms;
+  p->npr = D.num_pr;
+  p->nr = D.num_r;
+  p->M = D.num_moments;
+  p->L1 = D.num_legendre;
+  p->nmat = D.num_materials;
+  p->G = D.global_num_groups;
+  p->sdoms.assign(D.sdoms, D.sdoms + D.num_sdoms);
+  p->prs.assign(D.pr, D.pr + D.num_pr);
+  p->rs.assign(D.r, D.r + D.num_r);
+  p->sigs = D.sigs;
+  p->m2l = D.moment_to_legendre;
+  auto bad = [&](const std::string &m) {
+    kr_problem_destroy(p);
+    return fail(KR_ERR_INVALID, "kr_problem_create: " + m);
+  };
+
+  // --- validate subdomains ------------------------------------------------
+  const kr_sdom_desc &s0 = p->sdoms[0];
+  std::vector<int> pr_rep(p->npr, -1);
+  for (int s = 0; s < p->nsdom; ++s) {
+    const kr_sdom_desc &S = p->sdoms[s];
+    if (S.num_directions <= 0 || S.num_groups <= 0 || S.ni <= 0 || S.nj <= 0 || S.nk <= 0)
+      return bad("subdomain with an empty set");
+    if (S.num_directions != s0.num_directions || S.num_groups != s0.num_groups || S.ni != s0.ni ||
+        S.nj != s0.nj || S.nk != s0.nk)
+      return bad("subdomains must share one shape");
+    if (S.pr_chunk < 0 || S.pr_chunk >= p->npr || S.r_chunk < 0 || S.r_chunk >= p->nr)
+      return bad("chunk id out of range");
+    if (!S.psi || !S.rhs || !S.i_plane || !S.j_plane || !S.k_plane || !S.xcos || !S.ycos || !S.zcos || !S.w ||
+        !S.id || !S.jd || !S.kd || !S.ell || !S.ell_plus || !S.dx || !S.dy || !S.dz)
+      return bad("missing field pointer");
+    for (int dim = 0; dim < 3; ++dim) {
+      int u = S.upwind[dim], w = S.downwind[dim];
+      if (u >= p->nsdom || w >= p->nsdom || u < -1 || w < -1) return bad("adjacency out of range");
+      if (u >= 0 && p->sdoms[u].downwind[dim] != s) return bad("upwind/downwind tables disagree");
+    }
+    if ((int64_t)S.num_directions * S.num_groups * S.ni * S.nj * S.nk > (int64_t)INT32_MAX)
+      return bad("subdomain chunk exceeds 2^31 elements");
+    if (pr_rep[S.pr_chunk] < 0) pr_rep[S.pr_chunk] = s;
+  }
+  for (int c = 0; c < p->npr; ++c) {
+    if (pr_rep[c] < 0) return bad("PR chunk without a subdomain");
+    if (!p->prs[c].phi || !p->prs[c].phi_out || !p->prs[c].sigt_zonal) return bad("missing PR field pointer");
+  }
+  for (int c = 0; c < p->nr; ++c) {
+    const kr_r_desc &R = p->rs[c];
+    if (!R.volume || !R.zone_to_mixelem || !R.zone_to_num_mixelem || !R.mixelem_to_zone ||
+        !R.mixelem_to_material || !R.mixelem_to_fraction || R.num_mixelem <= 0)
+      return bad("missing zone-set field pointer");
+  }
+
+  // --- host copies of small tables (quadrature, mesh, sigs) --------------
+  const int nd = s0.num_directions, ng = s0.num_groups, ni = s0.ni, nj = s0.nj, nk = s0.nk;
+  const int64_t nz = (int64_t)ni * nj * nk;
+  // 2*cos/delta tables, refreshed on device before each sweep
+  const int64_t coef_per = (int64_t)nd * (ni + nj + nk);
+  std::vector<double> coef(coef_per * p->nsdom, 0.0);
+  p->h_sd.resize(p->nsdom);
+  int rc = KR_OK;
+  for (int s = 0; s < p->nsdom; ++s) {
+    const kr_sdom_desc &S = p->sdoms[s];
+    int32_t sgn[3];
+    std::vector<int32_t> tmp;
+    if ((rc = download(ctx, tmp, S.id, 1))) { kr_problem_destroy(p); return rc; }
+    sgn[0] = tmp[0];
+    if ((rc = download(ctx, tmp, S.jd, 1))) { kr_problem_destroy(p); return rc; }
+    sgn[1] = tmp[0];
+    if ((rc = download(ctx, tmp, S.kd, 1))) { kr_problem_destroy(p); return rc; }
+    sgn[2] = tmp[0];
+
+    DevSdom &H = p->h_sd[s];
+    memset(&H, 0, sizeof(H));
+    H.psi = S.psi;
+    H.rhs = S.rhs;
+    H.plane[0] = S.i_plane;
+    H.plane[1] = S.j_plane;
+    H.plane[2] = S.k_plane;
+    H.sig = p->prs[S.pr_chunk].sigt_zonal;
+    H.cosines[0] = S.xcos;
+    H.cosines[1] = S.ycos;
+    H.cosines[2] = S.zcos;
+    H.delta[0] = S.dx;
+    H.delta[1] = S.dy;
+    H.delta[2] = S.dz;
+    H.w = S.w;
+    H.ell = S.ell;
+    H.ell_plus = S.ell_plus;
+    H.volume = p->rs[S.r_chunk].volume;
+    H.nd = nd; H.ng = ng; H.ni = ni; H.nj = nj; H.nk = nk; H.nz = (int32_t)nz;
+    H.id = sgn[0]; H.jd = sgn[1]; H.kd = sgn[2];
+    H.pr = S.pr_chunk; H.r = S.r_chunk; H.gset = S.gset; H.dset = S.dset; H.glo = S.group_lower;
+    for (int dim = 0; dim < 3; ++dim) { H.up[dim] = S.upwind[dim]; H.down[dim] = S.downwind[dim]; }
+    Str3 f = flux_strides(p->nesting, nd, ng, nz);
+    H.fa = f.a; H.fg = f.g; H.fz = f.z;
+    sigt_strides(p->nesting, ng, nz, &H.sg, &H.sz);
+    const int64_t faces[3] = {(int64_t)nj * nk, (int64_t)ni * nk, (int64_t)ni * nj};
+    for (int dim = 0; dim < 3; ++dim) {
+      Str3 ps = flux_strides(p->nesting, nd, ng, faces[dim]);
+      H.pa[dim] = ps.a; H.pg[dim] = ps.g; H.pf[dim] = ps.z;
+    }
+  }
+  rc = upload(ctx, &p->d_coef, coef);
+  if (rc) { kr_problem_destroy(p); return rc; }
+  for (int s = 0; s < p->nsdom; ++s) {
+    double *A = p->d_coef + coef_per * s;
+    p->h_sd[s].coef[0] = A;
+    p->h_sd[s].coef[1] = A + (int64_t)nd * ni;
+    p->h_sd[s].coef[2] = A + (int64_t)nd * (ni + nj);
+  }
+
+  // --- PR and R tables ------------------------------------------------------
+  std::vector<int32_t> pr_sdoms;
+  p->h_pr.resize(p->npr);
+  p->pr_gset.resize(p->npr);
+  p->pr_r.resize(p->npr);
+  for (int c = 0; c < p->npr; ++c) {
+    DevPR &H = p->h_pr[c];
+    memset(&H, 0, sizeof(H));
+    const kr_sdom_desc &S = p->sdoms[pr_rep[c]];
+    H.phi = p->prs[c].phi;
+    H.phi_out = p->prs[c].phi_out;
+    H.sig = p->prs[c].sigt_zonal;
+    H.ng = ng;
+    H.nz = (int32_t)nz;
+    H.r = S.r_chunk;
+    H.gset = S.gset;
+    H.glo = S.group_lower;
+    H.sdom_begin = (int32_t)pr_sdoms.size();
+    for (int s = 0; s < p->nsdom; ++s)
+      if (p->sdoms[s].pr_chunk == c) pr_sdoms.push_back(s);
+    H.sdom_count = (int32_t)pr_sdoms.size() - H.sdom_begin;
+    Str3 m = flux_strides(p->nesting, p->M, ng, nz);
+    H.ma = m.a; H.mg = m.g; H.mz = m.z;
+    p->pr_gset[c] = S.gset;
+    p->pr_r[c] = S.r_chunk;
+  }
+  std::vector<DevR> hr(p->nr);
+  p->max_gz = (int64_t)ng * nz;
+  p->max_gmix = 0;
+  for (int c = 0; c < p->nr; ++c) {
+    const kr_r_desc &R = p->rs[c];
+    DevR &H = hr[c];
+    H.zone_to_mixelem = R.zone_to_mixelem;
+    H.zone_to_num = R.zone_to_num_mixelem;
+    H.mix_zone = R.mixelem_to_zone;
+    H.mix_mat = R.mixelem_to_material;
+    H.mix_frac = R.mixelem_to_fraction;
+    H.volume = R.volume;
+    H.num_mix = R.num_mixelem;
+    H.nz = (int32_t)nz;
+    p->max_gmix = std::max<int64_t>(p->max_gmix, (int64_t)ng * R.num_mixelem);
+  }
+
+  // --- sigs structure -------------------------------------------------------
+  {
+    const int64_t L1 = p->L1, G = p->G, nmat = p->nmat;
+    const bool n_first = (p->nesting == KR_DGZ || p->nesting == KR_DZG || p->nesting == KR_ZDG);
+    if (n_first) { p->sigs_str[0] = L1 * G * G; p->sigs_str[1] = G * G; p->sigs_str[2] = G; p->sigs_str[3] = 1; }
+    else { p->sigs_str[0] = G * G * L1; p->sigs_str[1] = 1; p->sigs_str[2] = G * L1; p->sigs_str[3] = L1; }
+    (void)nmat;
+    std::vector<int32_t> flags(SIGS_FLAG_BLOCKS, 0);
+    if ((rc = upload(ctx, &p->d_sigs_flags, flags))) { kr_problem_destroy(p); return rc; }
+  }
+
+  // --- sweep planes tables --------------------------------------------------
+  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom);
+  for (int s = 0; s < p->nsdom; ++s) {
+    const DevSdom &H = p->h_sd[s];
+    for (int dim = 0; dim < 3; ++dim) {
+      pref[s].in[dim] = H.plane[dim];
+      pref[s].out[dim] = H.plane[dim];
+      ppipe[s].in[dim] = H.up[dim] >= 0 ? H.plane[dim] : nullptr;
+      ppipe[s].out[dim] = H.down[dim] >= 0 ? p->h_sd[H.down[dim]].plane[dim] : H.plane[dim];
+    }
+  }
+
+  // --- sweep schedules --------------------------------------------------------
+  p->shape_ref = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 1);
+  p->shape_pipe = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 0);
+  const int32_t pgs = (int32_t)(((int64_t)nd * ng) / p->shape_pipe.Q);
+  std::vector<SweepUnit> uref, upipe;
+  {
+    const int32_t R = pick_R(pgs, p->shape_ref);
+    for (int s = 0; s < p->nsdom; ++s) {
+      p->ref_unit_begin.push_back((int32_t)uref.size());
+      append_units(uref, s, pgs, R);
+      p->ref_unit_count.push_back((int32_t)uref.size() - p->ref_unit_begin.back());
+    }
+    std::vector<int32_t> all(p->nsdom);
+    for (int s = 0; s < p->nsdom; ++s) all[s] = s;
+    std::vector<std::vector<int32_t>> levels;
+    if ((rc = sweep_levels(p, all, levels))) { kr_problem_destroy(p); return rc; }
+    for (auto &lv : levels) {
+      const int32_t Rl = pick_R((int64_t)pgs * lv.size(), p->shape_pipe);
+      p->pipe_level_begin.push_back((int32_t)upipe.size());
+      for (int32_t s : lv) append_units(upipe, s, pgs, Rl);
+      p->pipe_level_count.push_back((int32_t)upipe.size() - p->pipe_level_begin.back());
+    }
+  }
+
+  std::vector<DevSdom> hsd = p->h_sd;
+  std::vector<char> popb;
+  p->pop_nblocks = population_block_table(p->h_sd.data(), p->nsdom, popb);
+  if ((rc = upload(ctx, &p->d_sd, hsd)) || (rc = upload(ctx, &p->d_pr, p->h_pr)) || (rc = upload(ctx, &p->d_r, hr)) ||
+      (rc = upload(ctx, &p->d_pr_sdoms, pr_sdoms)) || (rc = upload(ctx, &p->d_planes_ref, pref)) ||
+      (rc = upload(ctx, &p->d_planes_pipe, ppipe)) || (rc = upload(ctx, &p->d_units_ref, uref)) ||
+      (rc = upload(ctx, &p->d_units_pipe, upipe))) {
+    kr_problem_destroy(p);
+    return rc;
+  }
+  {
+    char *db = nullptr;
+    if ((rc = upload(ctx, &db, popb))) { kr_problem_destroy(p); return rc; }
+    p->d_pop_blocks = db;
+    if (hipMalloc((void **)&p->d_pop_partial, sizeof(double) * std::max<int64_t>(1, p->pop_nblocks)) != hipSuccess ||
+        hipMalloc((void **)&p->d_pop_out, sizeof(double)) != hipSuccess) {
+      kr_problem_destroy(p);
+      return fail(KR_ERR_NOMEM, "kr_problem_create: population buffers");
+    }
+  }
+  *out = p;
+  return KR_OK;
+}
+
+int kr_problem_destroy(kr_problem *p) {
+  if (!p) return KR_OK;
+  if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+  for (auto &tc : p->pending) {
+    hipEventDestroy(tc.start);
+    hipEventDestroy(tc.stop);
+  }
+  void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
+                  p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
+                  p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  delete p;
+  return KR_OK;
+}
+
+int kr_kconst(kr_context *ctx, double *chunk, int64_t n, double value) {
+  if (!ctx || (!chunk && n > 0) || n < 0) return fail(KR_ERR_INVALID, "kr_kconst: bad argument");
+  return launch_fill(ctx->stream, chunk, n, value);
+}
+
+int kr_kcopy(kr_context *ctx, double *dst, const double *src, int64_t n) {
+  if (!ctx || ((!dst || !src) && n > 0) || n < 0) return fail(KR_ERR_INVALID, "kr_kcopy: bad argument");
+  return launch_copy(ctx->stream, dst, src, n);
+}
+
+int kr_ltimes(kr_problem *p, int32_t accumulate) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_ltimes: null problem");
+  CallTimer t(p, "LTimes");
+  return launch_ltimes(launch_ctx(p), accumulate ? 1 : 0);
+}
+
+int kr_lplustimes(kr_problem *p, int32_t accumulate) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_lplustimes: null problem");
+  CallTimer t(p, "LPlusTimes");
+  return launch_lplustimes(launch_ctx(p), accumulate ? 1 : 0);
+}
+
+int kr_scattering(kr_problem *p, int32_t accumulate) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_scattering: null problem");
+  CallTimer t(p, "Scattering");
+  return launch_scattering(launch_ctx(p), p->pr_gset.data(), p->pr_r.data(), accumulate ? 1 : 0);
+}
+
+int kr_source(kr_problem *p, double strength) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_source: null problem");
+  CallTimer t(p, "Source");
+  return launch_source(launch_ctx(p), strength);
+}
+
+int kr_sweep_subdomain(kr_problem *p, int32_t sdom) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_sweep_subdomain: null problem");
+  if (sdom < 0 || sdom >= p->nsdom) return fail(KR_ERR_INVALID, "kr_sweep_subdomain: bad subdomain id");
+  CallTimer t(p, "SweepSubdomain");
+  int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, sdom, 1);
+  if (rc) return rc;
+  return launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_ref, p->d_units_ref + p->ref_unit_begin[sdom],
+                      p->ref_unit_count[sdom], p->shape_ref);
+}
+
+int kr_sweep_solver(kr_problem *p, const int32_t *list, int32_t num) {
+  if (!p || (!list && num > 0) || num < 0) return fail(KR_ERR_INVALID, "kr_sweep_solver: bad argument");
+  CallTimer t(p, "SweepSolver");
+  {
+    int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, 0, p->nsdom);
+    if (rc) return rc;
+  }
+  bool all = (num == p->nsdom);
+  for (int32_t i = 0; all && i < num; ++i) all = (list[i] == i);
+  if (all) {
+    for (size_t lv = 0; lv < p->pipe_level_begin.size(); ++lv) {
+      CallTimer tk(p, "k_sweep");  // one sweep-kernel launch (one zone-set stage)
+      int rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
+                            p->pipe_level_count[lv], p->shape_pipe);
+      if (rc) return rc;
+    }
+    return KR_OK;
+  }
+  std::vector<int32_t> v(list, list + num);
+  std::vector<std::vector<int32_t>> levels;
+  int rc = sweep_levels(p, v, levels);
+  if (rc) return rc;
+  const int32_t pgs = (int32_t)(((int64_t)p->h_sd[0].nd * p->h_sd[0].ng) / p->shape_pipe.Q);
+  std::vector<SweepUnit> u;
+  std::vector<int32_t> lb, lc;
+  for (auto &lv : levels) {
+    const int32_t Rl = pick_R((int64_t)pgs * lv.size(), p->shape_pipe);
+    lb.push_back((int32_t)u.size());
+    for (int32_t s : lv) append_units(u, s, pgs, Rl);
+    lc.push_back((int32_t)u.size() - lb.back());
+  }
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  if ((int32_t)u.size() > p->units_tmp_cap) {
+    if (p->d_units_tmp) KR_HIP_CHECK(hipFree(p->d_units_tmp));
+    KR_HIP_CHECK(hipMalloc((void **)&p->d_units_tmp, u.size() * sizeof(SweepUnit)));
+    p->units_tmp_cap = (int32_t)u.size();
+  }
+  KR_HIP_CHECK(hipMemcpy(p->d_units_tmp, u.data(), u.size() * sizeof(SweepUnit), hipMemcpyHostToDevice));
+  for (size_t l = 0; l < lb.size(); ++l) {
+    rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_pipe, p->d_units_tmp + lb[l], lc[l], p->shape_pipe);
+    if (rc) return rc;
+  }
+  return KR_OK;
+}
+
+int kr_population(kr_problem *p, double *out) {
+  if (!p || !out) return fail(KR_ERR_INVALID, "kr_population: null argument");
+  {
+    CallTimer t(p, "Population");
+    int rc = launch_population(launch_ctx(p), p->d_pop_blocks, p->pop_nblocks, p->d_pop_partial, p->d_pop_out);
+    if (rc) return rc;
+  }
+  KR_HIP_CHECK(hipMemcpyAsync(out, p->d_pop_out, sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  return KR_OK;
+}
+
+int kr_problem_set_timing(kr_problem *p, int32_t enable) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_problem_set_timing: null problem");
+  p->timing = enable ? 1 : 0;
+  return KR_OK;
+}
+
+int kr_problem_timer(kr_problem *p, const char *name, double *seconds, int64_t *count) {
+  if (!p || !name) return fail(KR_ERR_INVALID, "kr_problem_timer: null argument");
+  int rc = flush_timers(p);
+  if (rc) return rc;
+  auto it = p->timers.find(name);
+  if (seconds) *seconds = it == p->timers.end() ? 0.0 : it->second.first;
+  if (count) *count = it == p->timers.end() ? 0 : it->second.second;
+  return KR_OK;
+}
+
+int kr_problem_timer_reset(kr_problem *p) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_problem_timer_reset: null problem");
+  int rc = flush_timers(p);
+  p->timers.clear();
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,45 @@
+// Launcher interface between the C-ABI layer and the kernel sources.
+#pragma once
+#include <vector>
+
+#include "kr_internal.h"
+
+namespace kr {
+
+// Everything a moment-space launch needs, resolved from a registered problem.
+struct LaunchCtx {
+  hipStream_t stream;
+  const DevSdom *sd;        // device
+  const DevPR *pr;          // device
+  const DevR *r;            // device
+  const int32_t *pr_sdoms;  // device
+  const double *sigs;       // device
+  const int64_t *m2l;       // device
+  int64_t sigs_str[4];      // mat, n, g, gp strides of data/sigs
+  int64_t sigs_dims[4];     // materials, legendre, groups, groups
+  int32_t *sigs_flags;      // device, SIGS_FLAG_BLOCKS ints: nonzero = a coupling entry seen
+  int num_pr, num_moments;
+  int64_t max_gz;           // max groups*zones over PR chunks
+  int64_t max_gmix;         // max groups*mixed elements over PR chunks
+};
+
+constexpr int SIGS_FLAG_BLOCKS = 64;
+
+int launch_ltimes(const LaunchCtx &L, int accumulate);
+int launch_lplustimes(const LaunchCtx &L, int accumulate);
+int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate);
+int launch_source(const LaunchCtx &L, double strength);
+int launch_population(const LaunchCtx &L, const void *blocks, int64_t num_blocks, double *partial, double *out);
+int64_t population_block_table(const DevSdom *host_sd, int nsdom, std::vector<char> &bytes);
+int launch_fill(hipStream_t st, double *p, int64_t n, double v);
+int launch_copy(hipStream_t st, double *d, const double *s, int64_t n);
+
+// Sweep: one launch sweeps every unit; units of one launch share `shape`.
+int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
+                 int32_t num_units, const SweepShape &shape);
+// Refresh the 2*cos/delta tables of subdomains [first, first+count).
+int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
+// Shape choice for a subdomain extent and nesting (pair lanes, patch).
+SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode);
+
+}  // namespace kr

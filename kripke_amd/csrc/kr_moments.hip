@@ -1,0 +1,400 @@
+// Moment-space kernels of the source iteration on gfx950:
+// LTimes, LPlusTimes, scattering, source, population, kConst/kCopy.
+//
+// All are HBM-streaming FP64 kernels.  One thread owns one (group, zone)
+// column and keeps its moments in registers; the direction loop reads the
+// quadrature tables through the scalar cache (they are wave-uniform).
+// Accumulations follow the reference order per output element (directions
+// ascending over subdomains ascending for LTimes, moments ascending for
+// LPlusTimes) with fused multiply-adds.
+#include "kr_launch.h"
+
+namespace kr {
+
+// Map a flat (group, zone) index to (g, z) so that consecutive threads are
+// consecutive in memory for the field's stride pattern.
+__device__ inline void gz_of(int64_t e, int32_t ng, int32_t nz, bool zfast, int32_t &g, int32_t &z) {
+  if (zfast) {
+    z = (int32_t)(e % nz);
+    g = (int32_t)(e / nz);
+  } else {
+    g = (int32_t)(e % ng);
+    z = (int32_t)(e / ng);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LTimes: src/Kripke/Kernel/LTimes.cpp:54-65 (body), :79-105 (sdom loop).
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
+                                                const int32_t *__restrict__ pr_sdoms, int accumulate) {
+  const DevPR &P = prs[blockIdx.y];
+  const int64_t n = (int64_t)P.ng * P.nz;
+  const DevSdom &S0 = sd[pr_sdoms[P.sdom_begin]];
+  const bool zfast = S0.fz <= S0.fg;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, P.ng, P.nz, zfast, g, z);
+    const int64_t po = g * P.mg + z * P.mz;
+    double acc[M];
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) acc[nm] = accumulate ? P.phi[po + nm * P.ma] : 0.0;
+    for (int k = 0; k < P.sdom_count; ++k) {
+      const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+      const double *__restrict__ psi = S.psi + g * S.fg + z * S.fz;
+      const double *__restrict__ ell = S.ell;
+      const int64_t fa = S.fa;
+      int d = 0;
+      for (; d + 2 <= S.nd; d += 2) {
+        const double p0 = __builtin_nontemporal_load(psi + d * fa);
+        const double p1 = __builtin_nontemporal_load(psi + (d + 1) * fa);
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[d * M + nm], p0, acc[nm]);
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[(d + 1) * M + nm], p1, acc[nm]);
+      }
+      for (; d < S.nd; ++d) {
+        const double p0 = psi[d * fa];
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[d * M + nm], p0, acc[nm]);
+      }
+    }
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) P.phi[po + nm * P.ma] = acc[nm];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LPlusTimes: src/Kripke/Kernel/LPlusTimes.cpp:49-60, :68-92.
+// ---------------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
+                                                    const int32_t *__restrict__ pr_sdoms, int accumulate) {
+  const DevPR &P = prs[blockIdx.y];
+  const int64_t n = (int64_t)P.ng * P.nz;
+  const DevSdom &S0 = sd[pr_sdoms[P.sdom_begin]];
+  const bool zfast = S0.fz <= S0.fg;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, P.ng, P.nz, zfast, g, z);
+    const int64_t po = g * P.mg + z * P.mz;
+    double pho[M];
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) pho[nm] = P.phi_out[po + nm * P.ma];
+    for (int k = 0; k < P.sdom_count; ++k) {
+      const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+      double *__restrict__ rhs = S.rhs + g * S.fg + z * S.fz;
+      const double *__restrict__ elp = S.ell_plus;
+      const int64_t fa = S.fa;
+      for (int d = 0; d < S.nd; ++d) {
+        double acc = accumulate ? rhs[d * fa] : 0.0;
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) acc = __fma_rn(elp[d * M + nm], pho[nm], acc);
+        __builtin_nontemporal_store(acc, rhs + d * fa);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Scattering: src/Kripke/Kernel/Scattering.cpp:73-99.
+// sigs_z(n,g,gp) = sum over the zone's mixed elements of sigs*fraction, in
+// mixed-element order, product then sum (no fusion), as the reference.
+// ---------------------------------------------------------------------------
+struct SigsStrides {
+  int64_t mat, n, g, gp;
+};
+
+__device__ inline double sigs_zone(const double *__restrict__ sigs, const SigsStrides &ss, const DevR &R, int32_t z,
+                                   int64_t n, int64_t gg, int64_t ggp) {
+  const int64_t m0 = R.zone_to_mixelem[z];
+  const int32_t nmix = R.zone_to_num[z];
+  double s = 0.0;
+  for (int32_t q = 0; q < nmix; ++q) {
+    const int64_t mix = m0 + q;
+    const int64_t mat = R.mix_mat[mix];
+    const double t = sigs[mat * ss.mat + n * ss.n + gg * ss.g + ggp * ss.gp] * R.mix_frac[mix];
+    s = s + t;
+  }
+  return s;
+}
+
+// Structure of data/sigs, recomputed on every scattering call so that edits
+// of the field between calls are honoured: each block ORs "an entry with
+// n > 0 or g != gp is nonzero" over its share into flags[blockIdx.x].
+__global__ __launch_bounds__(256) void k_sigs_structure(const double *__restrict__ sigs, SigsStrides ss, int64_t nmat,
+                                                        int64_t L1, int64_t G, int32_t *__restrict__ flags) {
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  const int64_t n = nmat * L1 * G * G;
+  int mine = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t gp = e % G, g = (e / G) % G, nl = (e / (G * G)) % L1, m = e / (G * G * L1);
+    if ((nl > 0 || g != gp) && sigs[m * ss.mat + nl * ss.n + g * ss.g + gp * ss.gp] != 0.0) mine = 1;
+  }
+  if (mine) atomicOr(&any, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) flags[blockIdx.x] = any;
+}
+
+__device__ inline bool sigs_coupled(const int32_t *__restrict__ flags) {
+  int v = 0;
+  for (int b = 0; b < SIGS_FLAG_BLOCKS; ++b) v |= flags[b];
+  return v != 0;
+}
+
+// Structure-aware path for isotropic, group-diagonal sigs (the generated
+// data, src/Kripke/Generate/Data.cpp:87-101): every skipped product is an
+// exact zero, so the result equals the dense loop bit for bit.
+__global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
+                                                      const double *__restrict__ sigs, SigsStrides ss, int M,
+                                                      int accumulate, const int32_t *__restrict__ flags) {
+  if (sigs_coupled(flags)) return;
+  const DevPR &P = prs[blockIdx.y];
+  const DevR &R = rs[P.r];
+  const int64_t n = (int64_t)P.ng * P.nz;
+  const bool zfast = P.mz <= P.mg;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, P.ng, P.nz, zfast, g, z);
+    const int64_t gg = P.glo + g;
+    const double s = sigs_zone(sigs, ss, R, z, 0, gg, gg);
+    const int64_t po = g * P.mg + z * P.mz;
+    const double base0 = accumulate ? P.phi_out[po] : 0.0;
+    const double t = s * P.phi[po];
+    P.phi_out[po] = base0 + t;
+    for (int nm = 1; nm < M; ++nm) {
+      const double b = accumulate ? P.phi_out[po + nm * P.ma] : 0.0;
+      P.phi_out[po + nm * P.ma] = b + 0.0;
+    }
+  }
+}
+
+// General sigs (group coupling, anisotropic moments): one thread per
+// (moment, group, zone) of the destination chunk, source groups in order.
+__global__ __launch_bounds__(256) void k_scatter_dense(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
+                                                       const double *__restrict__ sigs, SigsStrides ss,
+                                                       const int64_t *__restrict__ m2l, int M, int src, int dst,
+                                                       int accumulate, const int32_t *__restrict__ flags) {
+  if (!sigs_coupled(flags)) return;
+  const DevPR &Ps = prs[src];
+  const DevPR &Pd = prs[dst];
+  const DevR &R = rs[Pd.r];
+  const int64_t n = (int64_t)M * Pd.ng * Pd.nz;
+  const bool zfast = Pd.mz <= Pd.mg;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t per = (int64_t)Pd.ng * Pd.nz;
+    const int nm = (int)(e / per);
+    int32_t g, z;
+    gz_of(e % per, Pd.ng, Pd.nz, zfast, g, z);
+    const int64_t nl = m2l[nm];
+    const int64_t od = nm * Pd.ma + g * Pd.mg + z * Pd.mz;
+    double acc = accumulate ? Pd.phi_out[od] : 0.0;
+    for (int32_t gp = 0; gp < Ps.ng; ++gp) {
+      const double s = sigs_zone(sigs, ss, R, z, nl, Pd.glo + g, Ps.glo + gp);
+      const double t = s * Ps.phi[nm * Ps.ma + gp * Ps.mg + z * Ps.mz];
+      acc = acc + t;
+    }
+    Pd.phi_out[od] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Source: src/Kripke/Kernel/Source.cpp:59-75.  A zone has at most one
+// material-0 mixed element, so the updates never collide.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_source(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
+                                                double strength) {
+  const DevPR &P = prs[blockIdx.y];
+  const DevR &R = rs[P.r];
+  const int64_t n = (int64_t)P.ng * R.num_mix;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t mix = e % R.num_mix;
+    const int32_t g = (int32_t)(e / R.num_mix);
+    if (R.mix_mat[mix] == 0) {
+      const int64_t z = R.mix_zone[mix];
+      double *p = P.phi_out + g * P.mg + z * P.mz;
+      *p = *p + strength * R.mix_frac[mix];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Population: src/Kripke/Kernel/Population.cpp:49-63.  Term order per
+// element is (w*psi)*volume; the sum is a fixed two-level tree, so the
+// result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+struct PopBlock {
+  int32_t sdom;
+  int32_t reserved;
+  int64_t begin, end;  // element range in the subdomain chunk (memory order)
+};
+
+__global__ __launch_bounds__(256) void k_pop_partial(const DevSdom *__restrict__ sd, const PopBlock *__restrict__ blocks,
+                                                     double *__restrict__ partial) {
+  __shared__ double red[256];
+  const PopBlock B = blocks[blockIdx.x];
+  const DevSdom &S = sd[B.sdom];
+  // memory order of psi: roles sorted by decreasing stride
+  const int64_t st[3] = {S.fa, S.fg, S.fz};
+  const int32_t ex[3] = {S.nd, S.ng, S.nz};
+  int o0 = 0, o1 = 1, o2 = 2;  // slow, mid, fast
+  if (st[o0] < st[o1]) { int t = o0; o0 = o1; o1 = t; }
+  if (st[o1] < st[o2]) { int t = o1; o1 = o2; o2 = t; }
+  if (st[o0] < st[o1]) { int t = o0; o0 = o1; o1 = t; }
+  const uint32_t nmid = ex[o1], nfast = ex[o2];
+  double acc = 0.0;
+  for (int64_t e = B.begin + threadIdx.x; e < B.end; e += blockDim.x) {
+    const uint32_t ue = (uint32_t)e;
+    const uint32_t f = ue % nfast, t = ue / nfast;
+    const uint32_t m = t % nmid, s = t / nmid;
+    uint32_t ix[3];
+    ix[o0] = s; ix[o1] = m; ix[o2] = f;
+    const double v = __builtin_nontemporal_load(S.psi + e);
+    acc += S.w[ix[0]] * v * S.volume[ix[2]];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_pop_final(const double *__restrict__ partial, int64_t n,
+                                                   double *__restrict__ out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) acc += partial[e];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// kConst / kCopy: src/Kripke/Kernel.h:37-81.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fill(double *__restrict__ p, int64_t n, double v) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = v;
+}
+
+__global__ __launch_bounds__(256) void k_copy(double *__restrict__ d, const double *__restrict__ s, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    d[e] = s[e];
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline int grid_for(int64_t n, int per_block = 256, int max_blocks = 8192) {
+  int64_t b = (n + per_block - 1) / per_block;
+  if (b > max_blocks) b = max_blocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+#define KR_MOMENT_CASES(X) X(1) X(4) X(9) X(16) X(25) X(36) X(49) X(64)
+
+int launch_ltimes(const LaunchCtx &L, int accumulate) {
+  dim3 grid(grid_for(L.max_gz), L.num_pr);
+  switch (L.num_moments) {
+#define X(m) \
+  case m: hipLaunchKernelGGL(k_ltimes<m>, grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
+    KR_MOMENT_CASES(X)
+#undef X
+    default: set_error("LTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
+  }
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_lplustimes(const LaunchCtx &L, int accumulate) {
+  dim3 grid(grid_for(L.max_gz), L.num_pr);
+  switch (L.num_moments) {
+#define X(m) \
+  case m: hipLaunchKernelGGL(k_lplustimes<m>, grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
+    KR_MOMENT_CASES(X)
+#undef X
+    default: set_error("LPlusTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
+  }
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate) {
+  SigsStrides ss{L.sigs_str[0], L.sigs_str[1], L.sigs_str[2], L.sigs_str[3]};
+  // 1) classify data/sigs as it is now; 2) the diagonal path and the dense
+  // path are both enqueued and each returns at once unless it applies.
+  hipLaunchKernelGGL(k_sigs_structure, dim3(SIGS_FLAG_BLOCKS), dim3(256), 0, L.stream, L.sigs, ss, L.sigs_dims[0],
+                     L.sigs_dims[1], L.sigs_dims[2], L.sigs_flags);
+  KR_HIP_CHECK(hipGetLastError());
+  dim3 grid(grid_for(L.max_gz), L.num_pr);
+  hipLaunchKernelGGL(k_scatter_diag, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments, accumulate,
+                     (const int32_t *)L.sigs_flags);
+  KR_HIP_CHECK(hipGetLastError());
+  // Dense: for each source chunk (work-list order) and each destination
+  // chunk on the same zone set (Scattering.cpp:137-145).
+  std::vector<int> first(L.num_pr, 1);
+  for (int s = 0; s < L.num_pr; ++s)
+    for (int d = 0; d < L.num_pr; ++d) {
+      if (pr_r[s] != pr_r[d]) continue;
+      const int acc = first[d] ? accumulate : 1;
+      first[d] = 0;
+      hipLaunchKernelGGL(k_scatter_dense, dim3(grid_for(L.max_gz * L.num_moments)), dim3(256), 0, L.stream, L.pr,
+                         L.r, L.sigs, ss, L.m2l, L.num_moments, s, d, acc, (const int32_t *)L.sigs_flags);
+      KR_HIP_CHECK(hipGetLastError());
+    }
+  (void)pr_gset;
+  return KR_OK;
+}
+
+int launch_source(const LaunchCtx &L, double strength) {
+  dim3 grid(grid_for(L.max_gmix), L.num_pr);
+  hipLaunchKernelGGL(k_source, grid, dim3(256), 0, L.stream, L.pr, L.r, strength);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_population(const LaunchCtx &L, const void *blocks, int64_t num_blocks, double *partial, double *out) {
+  hipLaunchKernelGGL(k_pop_partial, dim3((unsigned)num_blocks), dim3(256), 0, L.stream, L.sd,
+                     (const PopBlock *)blocks, partial);
+  KR_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_pop_final, dim3(1), dim3(256), 0, L.stream, partial, num_blocks, out);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int64_t population_block_table(const DevSdom *host_sd, int nsdom, std::vector<char> &bytes) {
+  std::vector<PopBlock> b;
+  const int64_t per_block = 256 * 16;
+  for (int s = 0; s < nsdom; ++s) {
+    const int64_t n = (int64_t)host_sd[s].nd * host_sd[s].ng * host_sd[s].nz;
+    for (int64_t e = 0; e < n; e += per_block) b.push_back(PopBlock{s, 0, e, std::min(n, e + per_block)});
+  }
+  bytes.assign((const char *)b.data(), (const char *)(b.data() + b.size()));
+  return (int64_t)b.size();
+}
+
+int launch_fill(hipStream_t st, double *p, int64_t n, double v) {
+  if (n <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256, 65536)), dim3(256), 0, st, p, n, v);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_copy(hipStream_t st, double *d, const double *s, int64_t n) {
+  if (n <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n, 256, 65536)), dim3(256), 0, st, d, s, n);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+}  // namespace kr

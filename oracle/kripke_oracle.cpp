@@ -1,0 +1,951 @@
+/*
+ * ORACLE — test infrastructure only (see kripke_oracle.h).
+ *
+ * CPU restatement of LLNL/Kripke's source-iteration path.  Every function
+ * cites the reference file:line it follows (paths relative to the reference
+ * repository root).  Loop orders per nesting follow the Sequential and
+ * OpenMP policies of src/Kripke/Arch/ headers.  Field memory orders follow
+ * DefaultOrder (src/Kripke/VarTypes.h:68-101) through VarLayout
+ * (src/Kripke/Core/VarLayout.h:87-135): the index type listed first in the
+ * nesting's DefaultOrder has the longest stride.
+ *
+ * Written in C-style C++ only because the reference's direction ordering
+ * is defined by std::sort with a non-strict comparator
+ * (src/Kripke/Generate/Quadrature.cpp:79-81,278); calling the same library
+ * routine reproduces its permutation.
+ */
+#include "kripke_oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef ORACLE_FMA
+#define ORACLE_FMA 0
+#endif
+
+/* c + a*b.  Reference build: rounded product then rounded sum. */
+static inline double madd(double a, double b, double c) {
+#if ORACLE_FMA
+  return std::fma(a, b, c);
+#else
+  return c + a * b;
+#endif
+}
+
+static thread_local std::string g_err;
+extern "C" const char *ko_error(void) { return g_err.c_str(); }
+
+static double now_s() {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Layout: strides of a (A, Group, Zone) field for nesting `layout`.        */
+/* A is Direction (Field_Flux) or Moment (Field_Moments); both sit before  */
+/* GlobalGroup/Group in the D-first lists and after Zone in the Z-first    */
+/* ones (src/Kripke/VarTypes.h:68-101), so the same rule serves both, and  */
+/* the plane fields (Direction, Group, ZoneX, ZoneY) with the face index   */
+/* playing the zone (src/Kripke/VarTypes.h:35-37).                          */
+/* ------------------------------------------------------------------------ */
+enum { DGZ = 0, DZG, GDZ, GZD, ZDG, ZGD };
+
+struct Str3 {
+  int64_t a, g, z;
+};
+
+static Str3 flux_strides(int layout, int64_t na, int64_t ng, int64_t nz) {
+  switch (layout) {
+    case DGZ: return {ng * nz, nz, 1};
+    case DZG: return {nz * ng, 1, ng};
+    case GDZ: return {nz, na * nz, 1};
+    case GZD: return {1, nz * na, na};
+    case ZDG: return {ng, 1, na * ng};
+    default:  return {1, na, ng * na}; /* ZGD */
+  }
+}
+
+/* Field_SigmaTZonal <Group, Zone>: [g][z] when GlobalGroup/Group precede Zone. */
+static void sigt_strides(int layout, int64_t ng, int64_t nz, int64_t *sg, int64_t *sz) {
+  bool g_first = (layout == DGZ || layout == GDZ || layout == GZD);
+  if (g_first) { *sg = nz; *sz = 1; } else { *sg = 1; *sz = ng; }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Problem state                                                            */
+/* ------------------------------------------------------------------------ */
+struct QuadPoint { /* src/Kripke/Generate/Quadrature.cpp:23-36 */
+  double xcos, ycos, zcos, w;
+  int id, jd, kd, octant;
+};
+
+struct Sdom {
+  int gs, ds, zx, zy, zz;
+  int64_t nd, ng, ni, nj, nk, nz;
+  int64_t dlo, glo, ilo, jlo, klo;
+  int pr, r;
+  int up[3], down[3];
+  int id, jd, kd;
+};
+
+struct RData {
+  std::vector<int> zone_to_mixelem, zone_to_num;
+  std::vector<int> mix_zone, mix_mat;
+  std::vector<double> mix_frac;
+  std::vector<double> volume;
+};
+
+struct Timer { double total = 0; int64_t count = 0; };
+
+struct ko_problem {
+  ko_input in;
+  int SP, SQ, Sx, Sy, Sz, nsdom, npr, nr;
+  int64_t M, L1, G, D;
+  std::vector<Sdom> sd;
+  std::vector<int> pr_rep;          /* work list of PR-space fields */
+  std::vector<QuadPoint> quad;
+  std::vector<std::vector<double>> ell, ell_plus; /* per dset, [d][nm] */
+  std::vector<std::vector<double>> xcos, ycos, zcos, w;
+  std::vector<std::vector<int>> qid, qjd, qkd;
+  std::vector<int> m2l;
+  std::vector<double> dx, dy, dz;   /* global per-axis */
+  std::vector<RData> rd;
+  double sigt_mat[3], sigs_mat[3];
+  std::vector<double> sigs;         /* [mat][n][g][gp] */
+  std::vector<std::vector<double>> psi, rhs, ipl, jpl, kpl;
+  std::vector<std::vector<double>> phi, phi_out, sigt;
+  Timer t_solve, t_lt, t_lpt, t_scat, t_src, t_ss, t_sweep, t_pop, t_gen;
+};
+
+/* ------------------------------------------------------------------------ */
+/* Generic loop nest: four index roles, loop levels in `ord`, outer `ncol`  */
+/* levels collapsed under OpenMP (the Collapse<omp_parallel_collapse_exec> */
+/* policies of the src/Kripke/Arch/ headers) or run sequentially.                   */
+/* ------------------------------------------------------------------------ */
+template <int O0, int O1, int O2, int O3, int NC, class F>
+static void nest_t(const int64_t ext[4], bool omp, F &body) {
+  const int64_t E0 = ext[O0], E1 = ext[O1], E2 = ext[O2], E3 = ext[O3];
+  if (NC == 3) {
+#pragma omp parallel for collapse(3) schedule(static) if (omp)
+    for (int64_t a = 0; a < E0; ++a)
+      for (int64_t b = 0; b < E1; ++b)
+        for (int64_t c = 0; c < E2; ++c) {
+          int64_t ix[4];
+          ix[O0] = a; ix[O1] = b; ix[O2] = c;
+          for (int64_t e = 0; e < E3; ++e) { ix[O3] = e; body(ix[0], ix[1], ix[2], ix[3]); }
+        }
+  } else {
+#pragma omp parallel for collapse(2) schedule(static) if (omp)
+    for (int64_t a = 0; a < E0; ++a)
+      for (int64_t b = 0; b < E1; ++b) {
+        int64_t ix[4];
+        ix[O0] = a; ix[O1] = b;
+        for (int64_t c = 0; c < E2; ++c) {
+          ix[O2] = c;
+          for (int64_t e = 0; e < E3; ++e) { ix[O3] = e; body(ix[0], ix[1], ix[2], ix[3]); }
+        }
+      }
+  }
+}
+
+/* Runtime order -> compile-time loop nest.  Every (order, collapse) pair used
+ * by the policy tables below has a case; an unknown pair aborts loudly. */
+#define NEST_CASE(a, b, c, d, n) \
+  case ((a * 64 + b * 16 + c * 4 + d) * 4 + n): nest_t<a, b, c, d, n>(ext, omp, body); return;
+template <class F>
+static void nest(const int64_t ext[4], const int ord[4], int ncol, bool omp, F body) {
+  switch ((ord[0] * 64 + ord[1] * 16 + ord[2] * 4 + ord[3]) * 4 + ncol) {
+    NEST_CASE(0, 1, 2, 3, 2) NEST_CASE(0, 1, 3, 2, 2) NEST_CASE(2, 0, 1, 3, 2) NEST_CASE(2, 3, 0, 1, 2)
+    NEST_CASE(3, 0, 1, 2, 2) NEST_CASE(3, 2, 0, 1, 2) NEST_CASE(0, 2, 1, 3, 2) NEST_CASE(0, 3, 1, 2, 2)
+    NEST_CASE(2, 3, 0, 1, 3) NEST_CASE(3, 2, 0, 1, 3) NEST_CASE(1, 2, 0, 3, 2) NEST_CASE(1, 2, 3, 0, 2)
+    NEST_CASE(3, 1, 2, 0, 2) NEST_CASE(0, 3, 1, 2, 3) NEST_CASE(1, 0, 2, 3, 2) NEST_CASE(1, 3, 2, 0, 2)
+    NEST_CASE(3, 0, 1, 2, 3)
+    default: fprintf(stderr, "oracle: no loop nest for this order\n"); abort();
+  }
+}
+#undef NEST_CASE
+
+struct Order { int ord[4]; int ncol; };
+
+/* src/Kripke/Arch/LTimes.h:20-114 (Sequential), :119-201 (OpenMP).  Roles nm,d,g,z. */
+static const Order LT_SEQ[6] = {{{0,1,2,3},2},{{0,1,3,2},2},{{2,0,1,3},2},{{2,3,0,1},2},{{3,0,1,2},2},{{3,2,0,1},2}};
+static const Order LT_OMP[6] = {{{0,2,1,3},2},{{0,3,1,2},2},{{2,0,1,3},2},{{2,3,0,1},3},{{3,0,1,2},2},{{3,2,0,1},2}};
+/* src/Kripke/Arch/LPlusTimes.h:21-118, :122-206.  Roles d,nm,g,z. */
+static const Order LP_SEQ[6] = {{{0,1,2,3},2},{{0,1,3,2},2},{{2,0,1,3},2},{{2,3,0,1},2},{{3,0,1,2},2},{{3,2,0,1},2}};
+static const Order LP_OMP[6] = {{{0,2,1,3},2},{{0,3,1,2},2},{{2,0,1,3},2},{{2,3,0,1},3},{{3,0,1,2},2},{{3,2,0,1},3}};
+/* src/Kripke/Arch/Scattering.h:20-119, :122-206.  Roles nm,g,gp,z. */
+static const Order SC_SEQ[6] = {{{0,1,2,3},2},{{0,3,1,2},2},{{1,2,0,3},2},{{1,2,3,0},2},{{3,0,1,2},2},{{3,1,2,0},2}};
+static const Order SC_OMP[6] = {{{0,1,2,3},2},{{0,3,1,2},3},{{1,0,2,3},2},{{1,3,2,0},2},{{3,0,1,2},3},{{3,1,2,0},2}};
+/* src/Kripke/Arch/Population.h:21-117, :119-208.  Roles d,g,z (+unit). */
+static const int POP_ORD[6][3] = {{0,1,2},{0,2,1},{1,0,2},{1,2,0},{2,0,1},{2,1,0}};
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature helpers: src/Kripke/Generate/Quadrature.cpp:46-183            */
+/* ------------------------------------------------------------------------ */
+static void gauss_legendre(double x1, double x2, std::vector<double> &x, std::vector<double> &w,
+                           double eps) { /* Quadrature.cpp:46-76 */
+  int n = (int)x.size();
+  int m = (n + 1) / 2;
+  double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 1; i <= m; i++) {
+    double z = cos(M_PI * (i - 0.25) / (n + 0.5)), z1, pp, p1, p2, p3;
+    do {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int j = 1; j <= n; j++) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+    } while (fabs(z - z1) > eps);
+    x[i - 1] = xm - xl * z;
+    x[n - i] = xm + xl * z;
+    w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - i] = w[i - 1];
+  }
+}
+
+static double fact_fcn(int n) { /* Quadrature.cpp:84-91 */
+  double f = 1.0;
+  for (int i = n; i > 0; --i) f *= (double)i;
+  return f;
+}
+
+static double pnm_fcn(int n, int m, double x) { /* Quadrature.cpp:93-138 */
+  double fact, pnn = 0, pmm, pmmp1, somx2;
+  pmm = 1.0;
+  if (m > 0) {
+    somx2 = sqrt((1.0 - x) * (1.0 + x));
+    fact = 1.0;
+    for (int i = 1; i <= m; i++) {
+      pmm *= -fact * somx2;
+      fact += 2.0;
+    }
+  }
+  if (n == m) return pmm;
+  pmmp1 = x * (2 * m + 1) * pmm;
+  if (n == (m + 1)) return pmmp1;
+  for (int nn = m + 2; nn <= n; nn++) {
+    pnn = (x * (2 * nn - 1) * pmmp1 - (nn + m - 1) * pmm) / (nn - m);
+    pmm = pmmp1;
+    pmmp1 = pnn;
+  }
+  return pnn;
+}
+
+static double ynm_fcn(int n, int m, double mu, double eta, double xi) { /* Quadrature.cpp:140-183 */
+  double floor_ = 1.e-20;
+  double tmp = fabs(eta / (mu + floor_));
+  double phi_tmp = atan(tmp), phi;
+  if ((mu > 0) && (eta > 0)) phi = phi_tmp;
+  else if ((mu < 0) && (eta > 0)) phi = M_PI - fabs(phi_tmp);
+  else if ((mu < 0) && (eta < 0)) phi = M_PI + fabs(phi_tmp);
+  else phi = 2.0 * M_PI - fabs(phi_tmp);
+  int am = std::abs(m);
+  double fac1 = fact_fcn(n - am), fac2 = fact_fcn(n + am);
+  double pnm = pnm_fcn(n, am, xi);
+  tmp = ((double)m) * phi;
+  double taum = (m >= 0) ? cos(tmp) : sin(-tmp);
+  double dm0 = (m == 0) ? 1.0 : 0.0;
+  tmp = ((2 * n + 1) * fac1) / (2.0 * (1.0 + dm0) * M_PI * fac2);
+  double anm = sqrt(tmp);
+  return anm * pnm * taum;
+}
+
+static bool dir_sort(QuadPoint const &a, QuadPoint const &b) { return b.octant < a.octant; }
+
+/* src/Kripke/Generate/Quadrature.cpp:192-310 */
+static bool create_quadrature(const ko_input &in, std::vector<QuadPoint> &dirs) {
+  int D = in.num_directions, per_oct = D / 8;
+  dirs.assign(D, QuadPoint{0, 0, 0, 0, 0, 0, 0, 0});
+  int np = in.quad_num_polar, na = in.quad_num_azimuthal;
+  if (np > 0) {
+    if (np % 4 != 0 || na % 2 != 0 || np * na != D) {
+      g_err = "bad Gauss-Legendre quadrature shape";
+      return false;
+    }
+    std::vector<double> pc(np), pw(np);
+    gauss_legendre(-1.0, 1.0, pc, pw, DBL_EPSILON);
+    std::vector<double> az(na), aw(na);
+    double dangle = 2.0 * M_PI / ((double)na);
+    for (int i = 0; i < na; i++) {
+      az[i] = (i == 0) ? dangle / 2.0 : az[i - 1] + dangle;
+      aw[i] = dangle;
+    }
+    int d = 0;
+    for (int i = 0; i < np; i++)
+      for (int j = 0; j < na; j++) {
+        double xc = sqrt(1.0 - pc[i] * pc[i]) * cos(az[j]);
+        double yc = sqrt(1.0 - pc[i] * pc[i]) * sin(az[j]);
+        double zc = pc[i];
+        QuadPoint &q = dirs[d++];
+        q.w = pw[i] * aw[j];
+        q.id = (xc > 0.) ? 1 : -1;
+        q.jd = (yc > 0.) ? 1 : -1;
+        q.kd = (zc > 0.) ? 1 : -1;
+        q.octant = (q.id == -1 ? 1 : 0) + (q.jd == -1 ? 2 : 0) + (q.kd == -1 ? 4 : 0);
+        q.xcos = std::abs(xc);
+        q.ycos = std::abs(yc);
+        q.zcos = std::abs(zc);
+      }
+    std::sort(dirs.begin(), dirs.end(), dir_sort);
+  } else {
+    double mu = cos(M_PI / 4);
+    double eta = sqrt(1 - mu * mu) * cos(M_PI / 4);
+    double xi = sqrt(1 - mu * mu) * sin(M_PI / 4);
+    int d = 0;
+    for (int oct = 0; oct < 8; ++oct) {
+      double om0 = oct & 0x1, om1 = (oct >> 1) & 0x1, om2 = (oct >> 2) & 0x1;
+      for (int sd = 0; sd < per_oct; sd++, d++) {
+        dirs[d].id = (om0 > 0.) ? 1 : -1;
+        dirs[d].jd = (om1 > 0.) ? 1 : -1;
+        dirs[d].kd = (om2 > 0.) ? 1 : -1;
+        dirs[d].w = 4.0 * M_PI / (double)D;
+        dirs[d].xcos = mu;
+        dirs[d].ycos = eta;
+        dirs[d].zcos = xi;
+        dirs[d].octant = oct ^ 7; /* left unset by the reference; not used by kernels */
+      }
+    }
+  }
+  return true;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Seeded generator of the configs: SplitMix64, boxes and cross sections.  */
+/* Same draw order in kripke_amd/csrc/host/generate.cpp.                   */
+/* ------------------------------------------------------------------------ */
+struct SplitMix64 {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+};
+
+struct Box { double lo[3], hi[3]; };
+
+static void seeded_geometry(uint64_t seed, double edge, Box boxes[2], double sigt[3], double sigs[3]) {
+  SplitMix64 rng{seed};
+  for (int b = 0; b < 2; ++b) /* b = 0: material 1's box, b = 1: material 0's box */
+    for (int ax = 0; ax < 3; ++ax) {
+      double side = edge * (0.1 + 0.4 * rng.uniform());
+      double lo = (edge - side) * rng.uniform();
+      boxes[b].lo[ax] = lo;
+      boxes[b].hi[ax] = lo + side;
+    }
+  for (int m = 0; m < 3; ++m) sigt[m] = pow(10.0, -4.0 + 4.0 * rng.uniform());
+  for (int m = 0; m < 3; ++m) sigs[m] = (0.3 + 0.6 * rng.uniform()) * sigt[m];
+}
+
+static bool in_box(const Box &b, double x, double y, double z) {
+  return x >= b.lo[0] && x <= b.hi[0] && y >= b.lo[1] && y <= b.hi[1] && z >= b.lo[2] && z <= b.hi[2];
+}
+
+/* Kobayashi-3i material function: src/Kripke/Generate/Space.cpp:142-177 */
+static int kobayashi(double x, double y, double z) {
+  x = std::abs(x);
+  y = std::abs(y);
+  z = std::abs(z);
+  if (x <= 10.0 && y <= 10.0 && z <= 10.0) return 0;
+  if (x <= 10.0 && y <= 60.0 && z <= 10.0) return 1;
+  if (x <= 40.0 && y >= 50.0 && y <= 60.0 && z <= 10.0) return 1;
+  if (x >= 30.0 && x <= 40.0 && y >= 50.0 && y <= 60.0 && z <= 40.0) return 1;
+  if (x >= 30.0 && x <= 40.0 && y >= 50.0 && z >= 30.0 && z <= 40.0) return 1;
+  return 2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* generateProblem: src/Kripke/Generate.cpp:24-94                            */
+/* ------------------------------------------------------------------------ */
+extern "C" ko_problem *ko_create(const ko_input *inp) {
+  const ko_input &in = *inp;
+  /* InputVariables::checkValues, src/Kripke/InputVariables.cpp:52-109 */
+  if (in.num_zonesets[0] <= 0 || in.num_zonesets[1] <= 0 || in.num_zonesets[2] <= 0 || in.num_groups < 1 ||
+      in.num_groupsets < 1 || in.num_groups % in.num_groupsets || in.num_directions < 8 ||
+      (in.num_dirsets % 8 && in.num_dirsets < 8) || in.num_directions % in.num_dirsets || in.legendre_order < 0 ||
+      in.layout < 0 || in.layout > 5) {
+    g_err = "invalid input variables";
+    return nullptr;
+  }
+  if (in.nx % in.num_zonesets[0] || in.ny % in.num_zonesets[1] || in.nz % in.num_zonesets[2]) {
+    g_err = "Number of zones must evenly divide into the number of subdomains"; /* Space.cpp:49,65,80 */
+    return nullptr;
+  }
+  double t0 = now_s();
+  ko_problem *P = new ko_problem();
+  P->in = in;
+  /* Decomp: src/Kripke/Generate/Decomp.cpp:23-56, PartitionSpace.cpp:75-123 */
+  P->SP = in.num_groupsets;
+  P->SQ = in.num_dirsets;
+  P->Sx = in.num_zonesets[0];
+  P->Sy = in.num_zonesets[1];
+  P->Sz = in.num_zonesets[2];
+  P->nsdom = P->SP * P->SQ * P->Sx * P->Sy * P->Sz;
+  P->npr = P->SP * P->Sx * P->Sy * P->Sz;
+  P->nr = P->Sx * P->Sy * P->Sz;
+  P->L1 = in.legendre_order + 1;
+  P->M = P->L1 * P->L1;
+  P->G = in.num_groups;
+  P->D = in.num_directions;
+
+  /* Quadrature (Quadrature.cpp:315-455) */
+  if (!create_quadrature(in, P->quad)) { delete P; return nullptr; }
+  int64_t nd = P->D / P->SQ, ng = P->G / P->SP;
+  int64_t ni = in.nx / P->Sx, nj = in.ny / P->Sy, nk = in.nz / P->Sz;
+  P->m2l.clear();
+  for (int n = 0; n < P->L1; ++n)
+    for (int m = -n; m <= n; ++m) P->m2l.push_back(n);
+  double SQRT4PI = std::sqrt(4 * M_PI);
+  P->ell.resize(P->SQ);
+  P->ell_plus.resize(P->SQ);
+  P->xcos.resize(P->SQ); P->ycos.resize(P->SQ); P->zcos.resize(P->SQ); P->w.resize(P->SQ);
+  P->qid.resize(P->SQ); P->qjd.resize(P->SQ); P->qkd.resize(P->SQ);
+  for (int q = 0; q < P->SQ; ++q) {
+    int64_t dlo = q * nd;
+    P->ell[q].assign(nd * P->M, 0.0);
+    P->ell_plus[q].assign(nd * P->M, 0.0);
+    for (int64_t d = 0; d < nd; ++d) {
+      const QuadPoint &pt = P->quad[d + dlo];
+      P->xcos[q].push_back(pt.xcos); P->ycos[q].push_back(pt.ycos);
+      P->zcos[q].push_back(pt.zcos); P->w[q].push_back(pt.w);
+      P->qid[q].push_back(pt.id); P->qjd[q].push_back(pt.jd); P->qkd[q].push_back(pt.kd);
+    }
+    int64_t nm = 0;
+    for (int n = 0; n < P->L1; n++)
+      for (int m = -n; m <= n; m++) {
+        for (int64_t d = 0; d < nd; ++d) {
+          const QuadPoint &pt = P->quad[d + dlo];
+          double xc = (pt.id) * (pt.xcos), yc = (pt.jd) * (pt.ycos), zc = (pt.kd) * (pt.zcos);
+          double ynm = ynm_fcn(n, m, xc, yc, zc);
+          P->ell[q][d * P->M + nm] = pt.w * ynm / SQRT4PI;     /* Field_Ell [d][nm] */
+          P->ell_plus[q][d * P->M + nm] = ynm * SQRT4PI;       /* Field_EllPlus [d][nm] */
+        }
+        nm++;
+      }
+  }
+
+  /* Subdomains, chunk maps and adjacency (PartitionSpace.cpp:246-273, Quadrature.cpp:460-518) */
+  P->sd.resize(P->nsdom);
+  for (int s = 0; s < P->nsdom; ++s) {
+    Sdom &S = P->sd[s];
+    int rem = s;
+    S.zz = rem % P->Sz; rem /= P->Sz;
+    S.zy = rem % P->Sy; rem /= P->Sy;
+    S.zx = rem % P->Sx; rem /= P->Sx;
+    S.ds = rem % P->SQ; rem /= P->SQ;
+    S.gs = rem;
+    S.nd = nd; S.ng = ng; S.ni = ni; S.nj = nj; S.nk = nk; S.nz = ni * nj * nk;
+    S.dlo = S.ds * nd; S.glo = S.gs * ng;
+    S.ilo = S.zx * ni; S.jlo = S.zy * nj; S.klo = S.zz * nk;
+    S.r = (S.zx * P->Sy + S.zy) * P->Sz + S.zz;
+    S.pr = S.gs * P->nr + S.r;
+    S.id = P->quad[S.dlo].id; S.jd = P->quad[S.dlo].jd; S.kd = P->quad[S.dlo].kd;
+    int sweep[3] = {S.id, S.jd, S.kd};
+    int coord[3] = {S.zx, S.zy, S.zz}, extent[3] = {P->Sx, P->Sy, P->Sz};
+    for (int dim = 0; dim < 3; ++dim) {
+      int cu[3] = {coord[0], coord[1], coord[2]}, cd[3] = {coord[0], coord[1], coord[2]};
+      cu[dim] -= sweep[dim];
+      cd[dim] += sweep[dim];
+      auto id_of = [&](int c[3]) {
+        return (((S.gs * P->SQ + S.ds) * P->Sx + c[0]) * P->Sy + c[1]) * P->Sz + c[2];
+      };
+      S.up[dim] = (cu[dim] < 0 || cu[dim] >= extent[dim]) ? -1 : id_of(cu);
+      S.down[dim] = (cd[dim] < 0 || cd[dim] >= extent[dim]) ? -1 : id_of(cd);
+    }
+  }
+  P->pr_rep.resize(P->npr);
+  for (int s = 0; s < P->nsdom; ++s)
+    if (P->sd[s].ds == 0) P->pr_rep[P->sd[s].pr] = s;
+
+  /* Space: src/Kripke/Generate/Space.cpp:38-365 */
+  double xmin, xmax, ymin, ymax, zmin, zmax;
+  Box boxes[2];
+  if (in.geometry == 1) {
+    xmin = ymin = zmin = 0.0;
+    xmax = ymax = zmax = 100.0;
+    seeded_geometry(in.seed, 100.0, boxes, P->sigt_mat, P->sigs_mat);
+  } else {
+    xmin = -60.0; xmax = 60.0; ymin = -100.0; ymax = 100.0; zmin = -60.0; zmax = 60.0;
+    for (int m = 0; m < 3; ++m) { P->sigt_mat[m] = in.sigt[m]; P->sigs_mat[m] = in.sigs[m]; }
+  }
+  double dx = (xmax - xmin) / in.nx, dy = (ymax - ymin) / in.ny, dz = (zmax - zmin) / in.nz;
+  P->dx.assign(in.nx, dx);
+  P->dy.assign(in.ny, dy);
+  P->dz.assign(in.nz, dz);
+  double zone_volume = dx * dy * dz;
+  int ns = in.num_material_subsamples;
+  double sample_vol_frac = 1.0 / (double)(ns * ns * ns);
+  P->rd.resize(P->nr);
+  for (int r = 0; r < P->nr; ++r) {
+    int zz = r % P->Sz, zy = (r / P->Sz) % P->Sy, zx = r / (P->Sz * P->Sy);
+    double x0 = xmin + dx * (zx * ni), y0 = ymin + dy * (zy * nj), z0 = zmin + dz * (zz * nk);
+    int64_t nzn = ni * nj * nk;
+    std::vector<double> frac(3 * nzn, 0.0);
+    for (int64_t k = 0; k < nk; k++)
+      for (int64_t j = 0; j < nj; j++)
+        for (int64_t i = 0; i < ni; i++) {
+          int64_t zone = (k * nj + j) * ni + i;
+          double xi = x0 + dx * i, yi = y0 + dy * j, zi = z0 + dz * k;
+          for (int si = 0; si < ns; ++si)
+            for (int sj = 0; sj < ns; ++sj)
+              for (int sk = 0; sk < ns; ++sk) {
+                double x = xi + dx * (si + 1) / (ns + 1);
+                double y = yi + dy * (sj + 1) / (ns + 1);
+                double z = zi + dz * (sk + 1) / (ns + 1);
+                int mat;
+                if (in.geometry == 1)
+                  mat = in_box(boxes[1], x, y, z) ? 0 : (in_box(boxes[0], x, y, z) ? 1 : 2);
+                else
+                  mat = kobayashi(x, y, z);
+                frac[3 * zone + mat] += sample_vol_frac;
+              }
+        }
+    RData &R = P->rd[r];
+    R.zone_to_mixelem.resize(nzn);
+    R.zone_to_num.resize(nzn);
+    R.volume.assign(nzn, zone_volume);
+    for (int64_t z = 0; z < nzn; ++z) {
+      R.zone_to_mixelem[z] = (int)R.mix_zone.size();
+      int cnt = 0;
+      double zone_frac = 0.0;
+      for (int m = 0; m < 3; ++m)
+        if (frac[3 * z + m] > 0.0) {
+          R.mix_zone.push_back((int)z);
+          R.mix_mat.push_back(m);
+          R.mix_frac.push_back(frac[3 * z + m]);
+          zone_frac += frac[3 * z + m];
+          cnt++;
+        }
+      if (zone_frac != 1.0) { g_err = "Zone fraction wrong"; delete P; return nullptr; } /* Space.cpp:312 */
+      R.zone_to_num[z] = cnt;
+    }
+  }
+
+  /* sigt_zonal: Space.cpp:334-363 ; data/sigs: Data.cpp:80-101 */
+  P->sigt.resize(P->npr);
+  for (int c = 0; c < P->npr; ++c) {
+    const Sdom &S = P->sd[P->pr_rep[c]];
+    const RData &R = P->rd[S.r];
+    int64_t sg, sz;
+    sigt_strides(in.layout, S.ng, S.nz, &sg, &sz);
+    std::vector<double> &st = P->sigt[c];
+    st.assign(S.ng * S.nz, 0.0);
+    for (int64_t g = 0; g < S.ng; ++g)
+      for (size_t m = 0; m < R.mix_zone.size(); ++m) {
+        int64_t z = R.mix_zone[m];
+        st[g * sg + z * sz] += R.mix_frac[m] * P->sigt_mat[R.mix_mat[m]];
+      }
+  }
+  P->sigs.assign(3 * P->L1 * P->G * P->G, 0.0);
+  for (int mat = 0; mat < 3; ++mat)
+    for (int64_t g = 0; g < P->G; ++g) P->sigs[((mat * P->L1 + 0) * P->G + g) * P->G + g] = P->sigs_mat[mat];
+
+  /* Fields: src/Kripke/Generate/Data.cpp:34-71 */
+  P->psi.resize(P->nsdom); P->rhs.resize(P->nsdom);
+  P->ipl.resize(P->nsdom); P->jpl.resize(P->nsdom); P->kpl.resize(P->nsdom);
+  for (int s = 0; s < P->nsdom; ++s) {
+    const Sdom &S = P->sd[s];
+    int64_t n = S.nd * S.ng;
+    P->psi[s].assign(n * S.nz, 0.0);
+    P->rhs[s].assign(n * S.nz, 0.0);
+    P->ipl[s].assign(n * S.nj * S.nk, 0.0);
+    P->jpl[s].assign(n * S.ni * S.nk, 0.0);
+    P->kpl[s].assign(n * S.ni * S.nj, 0.0);
+  }
+  P->phi.resize(P->npr); P->phi_out.resize(P->npr);
+  for (int c = 0; c < P->npr; ++c) {
+    const Sdom &S = P->sd[P->pr_rep[c]];
+    P->phi[c].assign(P->M * S.ng * S.nz, 0.0);
+    P->phi_out[c].assign(P->M * S.ng * S.nz, 0.0);
+  }
+  P->t_gen.total += now_s() - t0;
+  P->t_gen.count++;
+  return P;
+}
+
+extern "C" void ko_destroy(ko_problem *p) { delete p; }
+extern "C" int32_t ko_num_sdoms(const ko_problem *p) { return p->nsdom; }
+extern "C" int32_t ko_num_pr(const ko_problem *p) { return p->npr; }
+extern "C" void ko_materials(const ko_problem *p, double *st, double *ss) {
+  for (int m = 0; m < 3; ++m) { st[m] = p->sigt_mat[m]; ss[m] = p->sigs_mat[m]; }
+}
+
+extern "C" int32_t ko_sdom_info(const ko_problem *p, int32_t s, int64_t *o) {
+  if (s < 0 || s >= p->nsdom) return -1;
+  const Sdom &S = p->sd[s];
+  int64_t v[26] = {S.gs, S.ds, S.zx, S.zy, S.zz, S.nd, S.ng, S.ni, S.nj, S.nk, S.dlo, S.glo, S.ilo,
+                   S.jlo, S.klo, S.pr, S.r, S.up[0], S.up[1], S.up[2], S.down[0], S.down[1], S.down[2],
+                   S.id, S.jd, S.kd};
+  memcpy(o, v, sizeof(v));
+  return 0;
+}
+
+template <class T>
+static int64_t copy_out(const std::vector<T> &v, double *dst) {
+  if (dst)
+    for (size_t i = 0; i < v.size(); ++i) dst[i] = (double)v[i];
+  return (int64_t)v.size();
+}
+template <class T>
+static int64_t copy_in(std::vector<T> &v, const double *src) {
+  for (size_t i = 0; i < v.size(); ++i) v[i] = (T)src[i];
+  return (int64_t)v.size();
+}
+
+#define KO_FIELDS(X)                                                                                     \
+  X("psi", p->psi, p->nsdom) X("rhs", p->rhs, p->nsdom) X("i_plane", p->ipl, p->nsdom)                 \
+  X("j_plane", p->jpl, p->nsdom) X("k_plane", p->kpl, p->nsdom) X("phi", p->phi, p->npr)               \
+  X("phi_out", p->phi_out, p->npr) X("sigt_zonal", p->sigt, p->npr) X("ell", p->ell, p->SQ)             \
+  X("ell_plus", p->ell_plus, p->SQ) X("xcos", p->xcos, p->SQ) X("ycos", p->ycos, p->SQ)                 \
+  X("zcos", p->zcos, p->SQ) X("w", p->w, p->SQ) X("id", p->qid, p->SQ) X("jd", p->qjd, p->SQ)          \
+  X("kd", p->qkd, p->SQ)
+
+extern "C" int64_t ko_get(const ko_problem *p, const char *name, int32_t chunk, double *dst) {
+  std::string n(name);
+#define X(NM, VEC, CNT) \
+  if (n == NM) { if (chunk < 0 || chunk >= (CNT)) return -1; return copy_out((VEC)[chunk], dst); }
+  KO_FIELDS(X)
+#undef X
+  if (n == "dx") return copy_out(p->dx, dst);
+  if (n == "dy") return copy_out(p->dy, dst);
+  if (n == "dz") return copy_out(p->dz, dst);
+  if (n == "sigs") return copy_out(p->sigs, dst);
+  if (n == "moment_to_legendre") return copy_out(p->m2l, dst);
+  if (chunk < 0 || chunk >= p->nr) return -1;
+  const RData &R = p->rd[chunk];
+  if (n == "volume") return copy_out(R.volume, dst);
+  if (n == "mixelem_to_zone") return copy_out(R.mix_zone, dst);
+  if (n == "mixelem_to_material") return copy_out(R.mix_mat, dst);
+  if (n == "mixelem_to_fraction") return copy_out(R.mix_frac, dst);
+  if (n == "zone_to_mixelem") return copy_out(R.zone_to_mixelem, dst);
+  if (n == "zone_to_num_mixelem") return copy_out(R.zone_to_num, dst);
+  return -1;
+}
+
+extern "C" int64_t ko_set(ko_problem *p, const char *name, int32_t chunk, const double *src) {
+  std::string n(name);
+#define X(NM, VEC, CNT) \
+  if (n == NM) { if (chunk < 0 || chunk >= (CNT)) return -1; return copy_in((VEC)[chunk], src); }
+  KO_FIELDS(X)
+#undef X
+  if (n == "sigs") return copy_in(p->sigs, src);
+  return -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernels                                                                  */
+/* ------------------------------------------------------------------------ */
+struct Scope {
+  Timer &t; double t0;
+  explicit Scope(Timer &tt) : t(tt), t0(now_s()) {}
+  ~Scope() { t.total += now_s() - t0; t.count++; }
+};
+
+/* kConst: src/Kripke/Kernel.h:37-55 (RAJA::loop_exec, always one host thread) */
+static void kconst(std::vector<double> &v, double value) {
+  for (size_t i = 0; i < v.size(); ++i) v[i] = value;
+}
+extern "C" void ko_kconst_all(ko_problem *p, const char *name, double value) {
+  std::string n(name);
+  auto all = [&](std::vector<std::vector<double>> &f) { for (auto &c : f) kconst(c, value); };
+  if (n == "psi") all(p->psi);
+  else if (n == "rhs") all(p->rhs);
+  else if (n == "phi") all(p->phi);
+  else if (n == "phi_out") all(p->phi_out);
+  else if (n == "i_plane") all(p->ipl);
+  else if (n == "j_plane") all(p->jpl);
+  else if (n == "k_plane") all(p->kpl);
+}
+
+/* LTimes: src/Kripke/Kernel/LTimes.cpp:79-105, body :54-65 */
+extern "C" void ko_ltimes(ko_problem *p, int omp) {
+  Scope sc(p->t_lt);
+  const int L = p->in.layout;
+  for (int s = 0; s < p->nsdom; ++s) {
+    const Sdom &S = p->sd[s];
+    const double *psi = p->psi[s].data();
+    double *phi = p->phi[S.pr].data();
+    const double *ell = p->ell[S.ds].data();
+    Str3 ps = flux_strides(L, S.nd, S.ng, S.nz), fs = flux_strides(L, p->M, S.ng, S.nz);
+    const int64_t M = p->M;
+    int64_t ext[4] = {M, S.nd, S.ng, S.nz};
+    const Order &o = omp ? LT_OMP[L] : LT_SEQ[L];
+    nest(ext, o.ord, o.ncol, omp != 0, [&](int64_t nm, int64_t d, int64_t g, int64_t z) {
+      double &f = phi[nm * fs.a + g * fs.g + z * fs.z];
+      f = madd(ell[d * M + nm], psi[d * ps.a + g * ps.g + z * ps.z], f);
+    });
+  }
+}
+
+/* LPlusTimes: src/Kripke/Kernel/LPlusTimes.cpp:68-92, body :49-60 */
+extern "C" void ko_lplustimes(ko_problem *p, int omp) {
+  Scope sc(p->t_lpt);
+  const int L = p->in.layout;
+  for (int s = 0; s < p->nsdom; ++s) {
+    const Sdom &S = p->sd[s];
+    double *rhs = p->rhs[s].data();
+    const double *po = p->phi_out[S.pr].data();
+    const double *elp = p->ell_plus[S.ds].data();
+    Str3 rs = flux_strides(L, S.nd, S.ng, S.nz), fs = flux_strides(L, p->M, S.ng, S.nz);
+    const int64_t M = p->M;
+    int64_t ext[4] = {S.nd, M, S.ng, S.nz};
+    const Order &o = omp ? LP_OMP[L] : LP_SEQ[L];
+    nest(ext, o.ord, o.ncol, omp != 0, [&](int64_t d, int64_t nm, int64_t g, int64_t z) {
+      double &r = rhs[d * rs.a + g * rs.g + z * rs.z];
+      r = madd(elp[d * M + nm], po[nm * fs.a + g * fs.g + z * fs.z], r);
+    });
+  }
+}
+
+/* scattering: src/Kripke/Kernel/Scattering.cpp:112-164, body :73-99 */
+extern "C" void ko_scattering(ko_problem *p, int omp) {
+  Scope sc(p->t_scat);
+  const int L = p->in.layout;
+  for (int cs = 0; cs < p->npr; ++cs)
+    for (int cd = 0; cd < p->npr; ++cd) {
+      const Sdom &Ss = p->sd[p->pr_rep[cs]], &Sd = p->sd[p->pr_rep[cd]];
+      if (Ss.r != Sd.r) continue; /* Scattering.cpp:137-145 */
+      const RData &R = p->rd[Ss.r];
+      const double *phi = p->phi[cs].data();
+      double *po = p->phi_out[cd].data();
+      Str3 fs = flux_strides(L, p->M, Ss.ng, Ss.nz), fd = flux_strides(L, p->M, Sd.ng, Sd.nz);
+      const int64_t G = p->G, L1 = p->L1, glo_s = Ss.glo, glo_d = Sd.glo;
+      const double *sigs = p->sigs.data();
+      const int *m2l = p->m2l.data();
+      int64_t ext[4] = {p->M, Sd.ng, Ss.ng, Ss.nz};
+      const Order &o = omp ? SC_OMP[L] : SC_SEQ[L];
+      nest(ext, o.ord, o.ncol, omp != 0, [&](int64_t nm, int64_t g, int64_t gp, int64_t z) {
+        int64_t n = m2l[nm];
+        int ms = R.zone_to_mixelem[z], me = ms + R.zone_to_num[z];
+        double sigs_z = 0.0;
+        for (int mix = ms; mix < me; ++mix) {
+          int mat = R.mix_mat[mix];
+          sigs_z += sigs[((mat * L1 + n) * G + (g + glo_d)) * G + (gp + glo_s)] * R.mix_frac[mix];
+        }
+        double &f = po[nm * fd.a + g * fd.g + z * fd.z];
+        f += sigs_z * phi[nm * fs.a + gp * fs.g + z * fs.z];
+      });
+    }
+}
+
+/* source: src/Kripke/Kernel/Source.cpp:83-115, body :59-75 */
+extern "C" void ko_source(ko_problem *p, int omp) {
+  Scope sc(p->t_src);
+  const int L = p->in.layout;
+  double strength = p->in.source_strength;
+  for (int c = 0; c < p->npr; ++c) {
+    const Sdom &S = p->sd[p->pr_rep[c]];
+    const RData &R = p->rd[S.r];
+    double *po = p->phi_out[c].data();
+    Str3 fs = flux_strides(L, p->M, S.ng, S.nz);
+    int64_t nmix = (int64_t)R.mix_zone.size();
+#pragma omp parallel for collapse(2) if (omp)
+    for (int64_t g = 0; g < S.ng; ++g)
+      for (int64_t mix = 0; mix < nmix; ++mix)
+        if (R.mix_mat[mix] == 0) {
+          int64_t z = R.mix_zone[mix];
+          po[0 * fs.a + g * fs.g + z * fs.z] += strength * R.mix_frac[mix];
+        }
+  }
+}
+
+/* sweepSubdomain: src/Kripke/Kernel/SweepSubdomain.cpp:26-124 */
+extern "C" void ko_sweep_subdomain(ko_problem *p, int32_t s, int omp) {
+  Scope sc(p->t_sweep);
+  const int L = p->in.layout;
+  const Sdom &S = p->sd[s];
+  const int64_t ni = S.ni, nj = S.nj, nk = S.nk;
+  const double *rhs = p->rhs[s].data();
+  double *psi = p->psi[s].data();
+  const double *sig = p->sigt[S.pr].data();
+  double *ipl = p->ipl[s].data(), *jpl = p->jpl[s].data(), *kpl = p->kpl[s].data();
+  const double *xc = p->xcos[S.ds].data(), *yc = p->ycos[S.ds].data(), *zc = p->zcos[S.ds].data();
+  const double *dx = p->dx.data() + S.ilo, *dy = p->dy.data() + S.jlo, *dz = p->dz.data() + S.klo;
+  Str3 fs = flux_strides(L, S.nd, S.ng, S.nz);
+  Str3 is = flux_strides(L, S.nd, S.ng, nj * nk), js = flux_strides(L, S.nd, S.ng, ni * nk),
+       ks = flux_strides(L, S.nd, S.ng, ni * nj);
+  int64_t sg, sz;
+  sigt_strides(L, S.ng, S.nz, &sg, &sz);
+  const int id = S.id, jd = S.jd, kd = S.kd; /* direction 0 of the sdom, :59-72 */
+  const int64_t i0 = id > 0 ? 0 : ni - 1, j0 = jd > 0 ? 0 : nj - 1, k0 = kd > 0 ? 0 : nk - 1;
+#pragma omp parallel for collapse(2) schedule(static) if (omp)
+  for (int64_t d = 0; d < S.nd; ++d)
+    for (int64_t g = 0; g < S.ng; ++g) {
+      for (int64_t kk = 0; kk < nk; ++kk) {
+        int64_t k = k0 + kd * kk;
+        for (int64_t jj = 0; jj < nj; ++jj) {
+          int64_t j = j0 + jd * jj;
+          for (int64_t ii = 0; ii < ni; ++ii) {
+            int64_t i = i0 + id * ii;
+            double xcos_dxi = 2.0 * xc[d] / dx[i];
+            double ycos_dyj = 2.0 * yc[d] / dy[j];
+            double zcos_dzk = 2.0 * zc[d] / dz[k];
+            int64_t z = (k * nj + j) * ni + i;
+            double &lf = ipl[d * is.a + g * is.g + (k * nj + j) * is.z];
+            double &fr = jpl[d * js.a + g * js.g + (k * ni + i) * js.z];
+            double &bo = kpl[d * ks.a + g * ks.g + (j * ni + i) * ks.z];
+            double num = madd(bo, zcos_dzk, madd(fr, ycos_dyj, madd(lf, xcos_dxi, rhs[d * fs.a + g * fs.g + z * fs.z])));
+            double psi_dgz = num / (xcos_dxi + ycos_dyj + zcos_dzk + sig[g * sg + z * sz]);
+            psi[d * fs.a + g * fs.g + z * fs.z] = psi_dgz;
+            lf = 2.0 * psi_dgz - lf;
+            fr = 2.0 * psi_dgz - fr;
+            bo = 2.0 * psi_dgz - bo;
+          }
+        }
+      }
+    }
+}
+
+/* SweepSolver with SweepComm on one rank: src/Kripke/SweepSolver.cpp:23-87,
+ * src/Kripke/ParallelComm.cpp:61-263, src/Kripke/ParallelComm/SweepComm.cpp:35-84 */
+extern "C" void ko_sweep_solver(ko_problem *p, int omp) {
+  Scope sc(p->t_ss);
+  for (auto &c : p->ipl) kconst(c, 0.0);
+  for (auto &c : p->jpl) kconst(c, 0.0);
+  for (auto &c : p->kpl) kconst(c, 0.0);
+  std::vector<int> q_ids, q_dep;
+  for (int s = 0; s < p->nsdom; ++s) { /* postRecvs: count on-rank upwind deps */
+    int nd = 0;
+    for (int dim = 0; dim < 3; ++dim)
+      if (p->sd[s].up[dim] >= 0) nd++;
+    q_ids.push_back(s);
+    q_dep.push_back(nd);
+  }
+  std::vector<std::vector<double>> *planes[3] = {&p->ipl, &p->jpl, &p->kpl};
+  while (!q_ids.empty()) {
+    int pick = -1;
+    for (size_t i = 0; i < q_ids.size(); ++i)
+      if (q_dep[i] == 0) { pick = (int)i; break; }
+    if (pick < 0) { g_err = "sweep dependency deadlock"; return; }
+    int s = q_ids[pick];
+    const Sdom &S = p->sd[s];
+    for (int dim = 0; dim < 3; ++dim)
+      if (S.up[dim] == -1) kconst((*planes[dim])[s], 0.0);
+    ko_sweep_subdomain(p, s, omp);
+    q_ids.erase(q_ids.begin() + pick); /* dequeueSubdomain */
+    q_dep.erase(q_dep.begin() + pick);
+    for (int dim = 0; dim < 3; ++dim) { /* postSends, on-rank branch */
+      int dn = S.down[dim];
+      if (dn < 0) continue;
+      for (size_t i = 0; i < q_ids.size(); ++i)
+        if (q_ids[i] == dn) { q_dep[i]--; break; }
+      (*planes[dim])[dn] = (*planes[dim])[s];
+    }
+  }
+}
+
+/* population: src/Kripke/Kernel/Population.cpp:74-101, body :49-63 */
+extern "C" void ko_population(ko_problem *p, int omp, double *literal, double *compensated) {
+  Scope sc(p->t_pop);
+  const int L = p->in.layout;
+  double part = 0.0, csum = 0.0, ccomp = 0.0;
+  for (int s = 0; s < p->nsdom; ++s) {
+    const Sdom &S = p->sd[s];
+    const double *psi = p->psi[s].data();
+    const double *w = p->w[S.ds].data();
+    const double *vol = p->rd[S.r].volume.data();
+    Str3 fs = flux_strides(L, S.nd, S.ng, S.nz);
+    int64_t ext[3] = {S.nd, S.ng, S.nz};
+    const int *o = POP_ORD[L];
+    double red = 0.0;
+    if (omp) {
+#pragma omp parallel for collapse(2) reduction(+ : red) schedule(static)
+      for (int64_t a = 0; a < ext[o[0]]; ++a)
+        for (int64_t b = 0; b < ext[o[1]]; ++b)
+          for (int64_t c = 0; c < ext[o[2]]; ++c) {
+            int64_t ix[3];
+            ix[o[0]] = a; ix[o[1]] = b; ix[o[2]] = c;
+            red += w[ix[0]] * psi[ix[0] * fs.a + ix[1] * fs.g + ix[2] * fs.z] * vol[ix[2]];
+          }
+    } else {
+      for (int64_t a = 0; a < ext[o[0]]; ++a)
+        for (int64_t b = 0; b < ext[o[1]]; ++b)
+          for (int64_t c = 0; c < ext[o[2]]; ++c) {
+            int64_t ix[3];
+            ix[o[0]] = a; ix[o[1]] = b; ix[o[2]] = c;
+            red += w[ix[0]] * psi[ix[0] * fs.a + ix[1] * fs.g + ix[2] * fs.z] * vol[ix[2]];
+          }
+    }
+    part += red;
+    if (compensated) { /* Neumaier sum over the same rounded terms, canonical order */
+      for (int64_t d = 0; d < S.nd; ++d)
+        for (int64_t g = 0; g < S.ng; ++g)
+          for (int64_t z = 0; z < S.nz; ++z) {
+            double t = w[d] * psi[d * fs.a + g * fs.g + z * fs.z] * vol[z];
+            double u = csum + t;
+            if (std::abs(csum) >= std::abs(t)) ccomp += (csum - u) + t;
+            else ccomp += (t - u) + csum;
+            csum = u;
+          }
+    }
+  }
+  if (literal) *literal = part; /* Comm::allReduceSumDouble is the identity on one rank */
+  if (compensated) *compensated = csum + ccomp;
+}
+
+extern "C" void ko_init_psi(ko_problem *p) {
+  Scope sc(p->t_solve);
+  for (auto &c : p->psi) kconst(c, 0.0); /* SteadyStateSolver.cpp:39 */
+}
+
+/* One iteration of src/Kripke/SteadyStateSolver.cpp:44-104 */
+extern "C" void ko_iterate(ko_problem *p, int omp, double *literal, double *compensated) {
+  Scope sc(p->t_solve);
+  for (auto &c : p->phi) kconst(c, 0.0);
+  ko_ltimes(p, omp);
+  for (auto &c : p->phi_out) kconst(c, 0.0);
+  ko_scattering(p, omp);
+  ko_source(p, omp);
+  for (auto &c : p->rhs) kconst(c, 0.0);
+  ko_lplustimes(p, omp);
+  ko_sweep_solver(p, omp);
+  ko_population(p, omp, literal, compensated);
+}
+
+extern "C" double ko_timer(const ko_problem *p, const char *name, int64_t *count) {
+  std::string n(name);
+  const Timer *t = nullptr;
+  if (n == "Solve") t = &p->t_solve;
+  else if (n == "LTimes") t = &p->t_lt;
+  else if (n == "LPlusTimes") t = &p->t_lpt;
+  else if (n == "Scattering") t = &p->t_scat;
+  else if (n == "Source") t = &p->t_src;
+  else if (n == "SweepSolver") t = &p->t_ss;
+  else if (n == "SweepSubdomain") t = &p->t_sweep;
+  else if (n == "Population") t = &p->t_pop;
+  else if (n == "Generate") t = &p->t_gen;
+  if (!t) { if (count) *count = -1; return -1.0; }
+  if (count) *count = t->count;
+  return t->total;
+}
+
+extern "C" void ko_timer_reset(ko_problem *p) {
+  p->t_solve = p->t_lt = p->t_lpt = p->t_scat = p->t_src = p->t_ss = p->t_sweep = p->t_pop = Timer();
+}
+
+extern "C" int32_t ko_omp_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

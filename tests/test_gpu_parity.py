@@ -1,0 +1,224 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Two oracle builds are used: 'fma' places fused multiply-adds exactly where
+the HIP kernels do, so every kernel except the population reduction must
+match it bit for bit; 'ref' has the reference's arithmetic (separate
+multiply and add), which the HIP path must match to 1e-12 relative
+(population, and phi normwise per moment), the bar of the task.
+"""
+import numpy as np
+import pytest
+
+from helpers import (canonical, kwargs_for, oracle_for, phi_parity, product_for, rel_normwise, seeded_fill)
+
+LAYOUTS = ["DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD"]
+TOL_REF = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _trio(ctx, **kw):
+    return product_for(ctx, **kw), oracle_for("fma", **kw), oracle_for("ref", **kw)
+
+
+def _assert_chunks(hp, of, orf, name, count, tol=1e-13):
+    for c in range(count):
+        a = hp.get(name, c)
+        b = of.get(name, c)
+        assert np.array_equal(a, b), "%s chunk %d differs from the fma oracle (max |d| %.3e)" % (
+            name, c, float(np.max(np.abs(a - b))))
+        if orf is not None:
+            assert rel_normwise(a, orf.get(name, c)) <= tol, name
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_ltimes(ctx, layout):
+    kw = kwargs_for(layout, gset=2, zset=(2, 1, 1), groups=8)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(11)
+    seeded_fill(hp, [of, orf], "psi", hp.num_sdoms, rng)
+    seeded_fill(hp, [of, orf], "phi", hp.num_pr, rng)
+    k = hp.kernels()
+    k.ltimes(accumulate=True)
+    of.ltimes()
+    orf.ltimes()
+    _assert_chunks(hp, of, orf, "phi", hp.num_pr)
+    # overwrite form == kConst(phi, 0) followed by LTimes
+    k.ltimes(accumulate=False)
+    of.kconst("phi", 0.0)
+    of.ltimes()
+    _assert_chunks(hp, of, None, "phi", hp.num_pr)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_lplustimes(ctx, layout):
+    kw = kwargs_for(layout, gset=2, zset=(1, 2, 1), groups=8)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(12)
+    seeded_fill(hp, [of, orf], "phi_out", hp.num_pr, rng)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng)
+    k = hp.kernels()
+    k.lplustimes(accumulate=True)
+    of.lplustimes()
+    orf.lplustimes()
+    _assert_chunks(hp, of, orf, "rhs", hp.num_sdoms)
+    k.lplustimes(accumulate=False)
+    of.kconst("rhs", 0.0)
+    of.lplustimes()
+    _assert_chunks(hp, of, None, "rhs", hp.num_sdoms)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_scattering_and_source(ctx, layout):
+    kw = kwargs_for(layout, gset=2, zset=(1, 1, 2), groups=8)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(13)
+    seeded_fill(hp, [of, orf], "phi", hp.num_pr, rng)
+    seeded_fill(hp, [of, orf], "phi_out", hp.num_pr, rng)
+    k = hp.kernels()
+    k.scattering(accumulate=True)
+    of.scattering()
+    orf.scattering()
+    _assert_chunks(hp, of, orf, "phi_out", hp.num_pr, tol=0.0)
+    k.source(1.0)
+    of.source()
+    orf.source()
+    _assert_chunks(hp, of, orf, "phi_out", hp.num_pr, tol=0.0)
+
+
+@pytest.mark.parametrize("layout", ["DGZ", "GZD", "ZDG"])
+def test_scattering_dense_sigs(ctx, layout):
+    """Group coupling and anisotropic moments: edits of data/sigs between
+    calls switch the kernel to the dense path."""
+    kw = kwargs_for(layout, gset=2, groups=4, legendre=1)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(14)
+    sigs = rng.uniform(0.0, 0.1, hp.get("data/sigs").size)
+    hp.set("data/sigs", 0, sigs)
+    # the oracle holds sigs as [mat][n][g][gp]; convert from the nesting's order
+    L1, G = 2, 4
+    n_first = layout in ("DGZ", "DZG", "ZDG")
+    s = sigs.reshape(3, L1, G, G) if n_first else np.transpose(sigs.reshape(3, G, G, L1), (0, 3, 1, 2))
+    for o in (of, orf):
+        o.set("sigs", 0, np.ascontiguousarray(s).ravel())
+    seeded_fill(hp, [of, orf], "phi", hp.num_pr, rng)
+    for o in (of, orf):
+        o.kconst("phi_out", 0.0)
+    hp.kernels().scattering(accumulate=False)
+    of.scattering()
+    orf.scattering()
+    _assert_chunks(hp, of, orf, "phi_out", hp.num_pr, tol=0.0)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_sweep_subdomain(ctx, layout):
+    """Kernel::sweepSubdomain on one subdomain: own planes in, own planes out."""
+    kw = kwargs_for(layout, zset=(2, 2, 1), groups=4, zones=(8, 6, 10))
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(15)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, scale=1.0, offset=2.0)
+    for name in ("i_plane", "j_plane", "k_plane"):
+        seeded_fill(hp, [of, orf], name, hp.num_sdoms, rng)
+    k = hp.kernels()
+    for s in (0, 5, hp.num_sdoms - 1):
+        k.sweep_subdomain(s)
+        of.sweep_subdomain(s)
+        orf.sweep_subdomain(s)
+    for name in ("psi", "i_plane", "j_plane", "k_plane"):
+        _assert_chunks(hp, of, orf, name, hp.num_sdoms, tol=1e-13)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_sweep_solver_pipeline(ctx, layout):
+    """On-device zone-set pipeline == SweepSolver's queue with plane hand-off."""
+    kw = kwargs_for(layout, zset=(2, 2, 2), groups=8, zones=(8, 8, 8))
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(16)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13)
+
+
+@pytest.mark.parametrize("layout", ["DGZ", "ZGD"])
+def test_population(ctx, layout):
+    kw = kwargs_for(layout, zset=(1, 2, 1), gset=2)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(17)
+    seeded_fill(hp, [of, orf], "psi", hp.num_sdoms, rng, offset=3.0)
+    g = hp.kernels().population()
+    lit, comp = orf.population()
+    assert abs(g - comp) <= 1e-13 * abs(comp)
+    assert g == hp.kernels().population(), "population must be deterministic"
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_iterations_c0(ctx, layout, mode):
+    """The CPU-runnable configuration (16^3 zones, 32 groups, 96 GL
+    directions 8x12, P4, 8 direction sets, 10 iterations): per-iteration
+    population within 1e-12 of the reference-arithmetic oracle's
+    compensated sum, phi within 1e-12 normwise per moment, and phi/psi bit
+    for bit equal to the fma-order oracle."""
+    kw = kwargs_for(layout, zones=(16, 16, 16), groups=32, quad=(8, 12), legendre=4)
+    hp, of, orf = _trio(ctx, **kw)
+    hp.init_psi()
+    of.init_psi()
+    orf.init_psi()
+    for it in range(10):
+        pg = hp.iterate(mode)
+        of.iterate()
+        _, comp = orf.iterate()
+        assert abs(pg - comp) <= TOL_REF * abs(comp), "iteration %d population %.17g vs %.17g" % (it, pg, comp)
+        norm, elem = phi_parity(hp, orf, hp.num_pr, layout, 25, 32, 16 ** 3)
+        assert norm <= TOL_REF, "iteration %d phi normwise %.3e" % (it, norm)
+        assert elem <= 1e-10, "iteration %d phi elementwise %.3e" % (it, elem)
+        for c in range(hp.num_pr):
+            assert np.array_equal(hp.get("phi", c), of.get("phi", c)), "iteration %d phi chunk %d" % (it, c)
+    for s in range(hp.num_sdoms):
+        assert np.array_equal(hp.get("psi", s), of.get("psi", s))
+
+
+def test_modes_and_decomposition(ctx):
+    """Reference call sequence (mode 0) and on-device pipeline (mode 1) give
+    identical phi; zone-set / group-set decompositions agree to rounding."""
+    base = kwargs_for("DGZ", zones=(8, 8, 8), groups=8)
+    a = product_for(ctx, **base)
+    b = product_for(ctx, **dict(base, zset=(2, 2, 2), gset=2))
+    a.init_psi()
+    b.init_psi()
+    for _ in range(4):
+        pa = a.iterate(1)
+        pb = b.iterate(0)
+        assert abs(pa - pb) <= 1e-13 * abs(pa)
+
+
+def test_reference_default_problem(ctx):
+    """The reference's own default run (Kobayashi-3i, dummy S2 with 96
+    directions, 32 groups in 2 group sets, P4), 3 iterations."""
+    kw = dict(zones=(16, 16, 16), groups=32, quad=None, directions=96, legendre=4, layout="DGZ", dset=8, gset=2,
+              zset=(1, 1, 1), geometry=0)
+    hp, of, orf = _trio(ctx, **kw)
+    hp.init_psi()
+    orf.init_psi()
+    for it in range(3):
+        pg = hp.iterate(1)
+        _, comp = orf.iterate()
+        assert abs(pg - comp) <= TOL_REF * abs(comp)
+
+
+def test_nesting_invariance_bitwise(ctx):
+    """psi after a sweep is bit-identical across the six nestings."""
+    ref = None
+    for layout in LAYOUTS:
+        hp = product_for(ctx, **kwargs_for(layout, zset=(2, 1, 1), groups=8))
+        hp.init_psi()
+        for _ in range(2):
+            hp.iterate(1)
+        psi = [canonical(hp.get("psi", s), layout, 2, 8, 4 * 8 * 8) for s in range(hp.num_sdoms)]
+        if ref is None:
+            ref = psi
+        else:
+            for x, y in zip(psi, ref):
+                assert np.array_equal(x, y), layout
