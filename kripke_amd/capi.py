@@ -4,6 +4,7 @@ Loads the in-tree library and fails loudly when it is missing: there is no
 CPU fallback behind this module.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -147,13 +148,17 @@ class Context:
 
     def __init__(self, device=0):
         self.h = P()
+        self._children = weakref.WeakSet()  # problems built on this context
         check(lib().kr_context_create(device, ctypes.byref(self.h)), "kr_context_create")
 
     def synchronize(self):
         check(lib().kr_synchronize(self.h), "kr_synchronize")
 
     def close(self):
+        """Destroys the problems still alive on this context, then the context."""
         if self.h:
+            for child in list(self._children):
+                child.close()
             lib().kr_context_destroy(self.h)
             self.h = None
 
@@ -168,8 +173,14 @@ class KernelProblem:
     """The registered kr_problem of a host problem: the Kernel:: entry points."""
 
     def __init__(self, handle, owner):
-        self.h = handle
+        self._h = handle
         self.owner = owner  # keeps the host problem alive
+
+    @property
+    def h(self):
+        if not self.owner.h:
+            raise KripkeError(1, "the host problem of this kernel problem is closed")
+        return self._h
 
     def ltimes(self, accumulate=True):
         check(lib().kr_ltimes(self.h, 1 if accumulate else 0), "kr_ltimes")
@@ -213,6 +224,10 @@ class HostProblem:
 
     def __init__(self, inp, ctx=None):
         self.ctx = ctx
+        if ctx is not None:
+            if not ctx.h:
+                raise KripkeError(1, "context is closed")
+            ctx._children.add(self)
         self.inp = inp
         self.h = P()
         check(lib().kr_host_create(ctx.h if ctx is not None else None, ctypes.byref(inp), ctypes.byref(self.h)),

@@ -68,6 +68,9 @@ FieldBase::FieldBase(kr_context *ctx, const Set &set, std::vector<Idx> idx, int 
   if (device && ctx) {
     if (kr_malloc(ctx, std::max<size_t>(m_total * m_elem, 8), &m_dev) != KR_OK)
       throw std::runtime_error(std::string("field allocation failed: ") + kr_last_error());
+    // chunks are padded to 256 bytes, so the slab is a whole number of doubles
+    if (kr_kconst(ctx, (double *)m_dev, (int64_t)(m_total * m_elem / sizeof(double)), 0.0) != KR_OK)
+      throw std::runtime_error(std::string("field zero-fill failed: ") + kr_last_error());
   }
 }
 

@@ -58,6 +58,7 @@ struct kr_problem {
   int64_t sigs_str[4] = {0, 0, 0, 0};
   int32_t *d_sigs_flags = nullptr;
   int64_t max_gz = 0, max_gmix = 0;
+  size_t coef_lds_bytes = 0;
   // timing
   int timing = 0;
   std::vector<TimedCall> pending;
@@ -111,6 +112,7 @@ LaunchCtx launch_ctx(kr_problem *p) {
   L.num_moments = p->M;
   L.max_gz = p->max_gz;
   L.max_gmix = p->max_gmix;
+  L.coef_lds_bytes = p->coef_lds_bytes;
   return L;
 }
 
@@ -155,7 +157,7 @@ void append_units(std::vector<SweepUnit> &u, int sdom, int32_t pgs, int32_t R) {
 }
 
 int32_t pick_R(int64_t total_pgs, const SweepShape &sh) {
-  const int nt = sh.Q * sh.PJ * sh.PK;
+  const int nt = sh.kind == 1 ? sh.RG * sh.PK : sh.Q * sh.PJ * sh.PK;
   const int64_t slots = 256LL * std::max(1, 1024 / std::max(nt, 1));  // resident workgroups, one per 1024 threads
   int64_t R = total_pgs / std::max<int64_t>(slots, 1);
   if (R < 1) R = 1;
@@ -429,6 +431,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     for (int s = 0; s < p->nsdom; ++s)
       if (p->sdoms[s].pr_chunk == c) pr_sdoms.push_back(s);
     H.sdom_count = (int32_t)pr_sdoms.size() - H.sdom_begin;
+    p->coef_lds_bytes = std::max(p->coef_lds_bytes, (size_t)H.sdom_count * nd * p->M * sizeof(double));
     Str3 m = flux_strides(p->nesting, p->M, ng, nz);
     H.ma = m.a; H.mg = m.g; H.mz = m.z;
     p->pr_gset[c] = S.gset;
@@ -450,6 +453,8 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     H.nz = (int32_t)nz;
     p->max_gmix = std::max<int64_t>(p->max_gmix, (int64_t)ng * R.num_mixelem);
   }
+
+  if (p->coef_lds_bytes > 65536) return bad("directions x moments per PR chunk exceed the 64 KiB LDS stage");
 
   // --- sigs structure -------------------------------------------------------
   {

@@ -9,6 +9,21 @@
 
 namespace kr {
 
+// Pointers into HBM as the global address space: loads and stores become
+// global_* instructions (counted on vmcnt only), not flat ones, so LDS waits
+// do not drain in-flight HBM traffic.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+
+template <class T>
+__device__ __forceinline__ const gmem<T> *gp(const T *p) {
+  return (const gmem<T> *)p;
+}
+template <class T>
+__device__ __forceinline__ gmem<T> *gp(T *p) {
+  return (gmem<T> *)p;
+}
+
 // Strides of a (A, Group, Zone) field, A = Direction or Moment, for a
 // nesting (src/Kripke/VarTypes.h:68-101).  Plane fields use the same rule
 // with the face index (slow*nfast + fast) in place of the zone.
@@ -93,10 +108,12 @@ struct SweepPlanes {
 
 // Per-launch sweep geometry (same for every unit of a launch).
 struct SweepShape {
+  int32_t kind;                 // 0: systolic pencil kernel, 1: row-scan kernel (zone-fastest nestings)
   int32_t Q, PJ, PK, NPJ, NPK;  // lanes per zone column, patch extents, patches per axis
+  int32_t RG;                   // row-scan kernel: lanes per row group
   int32_t inner_is_g;           // pair index = outer*inner + q ordering
   int32_t ref_mode;             // 1: read/write own planes only (Kernel::sweepSubdomain)
-  int32_t reserved;
+  int32_t nd, ni, nj, nk;       // subdomain extents
 };
 
 void set_error(const std::string &msg);

@@ -21,6 +21,7 @@ struct LaunchCtx {
   int num_pr, num_moments;
   int64_t max_gz;           // max groups*zones over PR chunks
   int64_t max_gmix;         // max groups*mixed elements over PR chunks
+  size_t coef_lds_bytes;    // ell/ell_plus of one PR chunk's subdomains
 };
 
 constexpr int SIGS_FLAG_BLOCKS = 64;
@@ -37,6 +38,9 @@ int launch_copy(hipStream_t st, double *d, const double *s, int64_t n);
 // Sweep: one launch sweeps every unit; units of one launch share `shape`.
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                  int32_t num_units, const SweepShape &shape);
+// Row-scan sweep of the zone-fastest nestings (kr_sweep_rows.hip).
+int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
+                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk);
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).
 int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // Shape choice for a subdomain extent and nesting (pair lanes, patch).

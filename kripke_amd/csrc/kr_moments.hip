@@ -23,75 +23,109 @@ __device__ inline void gz_of(int64_t e, int32_t ng, int32_t nz, bool zfast, int3
   }
 }
 
+// Coefficients of a PR chunk's subdomains staged in LDS: ell or ell_plus of
+// every contributing subdomain, [k][d][nm] (both fields are [d][nm]).
+// Wave-uniform addresses make the reads LDS broadcasts.
+__device__ inline void stage_coefficients(double *lds, const DevSdom *__restrict__ sd,
+                                          const int32_t *__restrict__ pr_sdoms, const DevPR &P, int M, bool plus) {
+  const int nd = sd[pr_sdoms[P.sdom_begin]].nd;
+  const int per = nd * M;
+  for (int k = 0; k < P.sdom_count; ++k) {
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+    const gmem<double> *src = gp(plus ? S.ell_plus : S.ell);
+    for (int e = threadIdx.x; e < per; e += blockDim.x) lds[k * per + e] = src[e];
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // LTimes: src/Kripke/Kernel/LTimes.cpp:54-65 (body), :79-105 (sdom loop).
+// A thread owns one (g,z) column: it issues the psi loads of a batch of DB
+// directions, then folds them into M accumulators, direction by direction.
 // ---------------------------------------------------------------------------
 template <int M>
 __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
                                                 const int32_t *__restrict__ pr_sdoms, int accumulate) {
+  constexpr int DB = 16;
+  extern __shared__ double coef[];
   const DevPR &P = prs[blockIdx.y];
+  stage_coefficients(coef, sd, pr_sdoms, P, M, false);
   const int64_t n = (int64_t)P.ng * P.nz;
   const DevSdom &S0 = sd[pr_sdoms[P.sdom_begin]];
   const bool zfast = S0.fz <= S0.fg;
+  const int nd = S0.nd;
+  const int64_t fa = S0.fa;
+  gmem<double> *phi = gp(P.phi);
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     int32_t g, z;
     gz_of(e, P.ng, P.nz, zfast, g, z);
     const int64_t po = g * P.mg + z * P.mz;
     double acc[M];
 #pragma unroll
-    for (int nm = 0; nm < M; ++nm) acc[nm] = accumulate ? P.phi[po + nm * P.ma] : 0.0;
+    for (int nm = 0; nm < M; ++nm) acc[nm] = accumulate ? phi[po + nm * P.ma] : 0.0;
     for (int k = 0; k < P.sdom_count; ++k) {
-      const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-      const double *__restrict__ psi = S.psi + g * S.fg + z * S.fz;
-      const double *__restrict__ ell = S.ell;
-      const int64_t fa = S.fa;
-      int d = 0;
-      for (; d + 2 <= S.nd; d += 2) {
-        const double p0 = __builtin_nontemporal_load(psi + d * fa);
-        const double p1 = __builtin_nontemporal_load(psi + (d + 1) * fa);
+      const gmem<double> *psi = gp(sd[pr_sdoms[P.sdom_begin + k]].psi) + g * S0.fg + z * S0.fz;
+      const double *ell = coef + k * nd * M;
+      for (int d0 = 0; d0 < nd; d0 += DB) {
+        double p[DB];
 #pragma unroll
-        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[d * M + nm], p0, acc[nm]);
+        for (int b = 0; b < DB; ++b)
+          if (d0 + b < nd) p[b] = __builtin_nontemporal_load(psi + (d0 + b) * fa);
 #pragma unroll
-        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[(d + 1) * M + nm], p1, acc[nm]);
-      }
-      for (; d < S.nd; ++d) {
-        const double p0 = psi[d * fa];
+        for (int b = 0; b < DB; ++b) {
+          if (d0 + b < nd) {
+            const double *eb = ell + (d0 + b) * M;
 #pragma unroll
-        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[d * M + nm], p0, acc[nm]);
+            for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(eb[nm], p[b], acc[nm]);
+          }
+        }
       }
     }
 #pragma unroll
-    for (int nm = 0; nm < M; ++nm) P.phi[po + nm * P.ma] = acc[nm];
+    for (int nm = 0; nm < M; ++nm) phi[po + nm * P.ma] = acc[nm];
   }
 }
 
 // ---------------------------------------------------------------------------
 // LPlusTimes: src/Kripke/Kernel/LPlusTimes.cpp:49-60, :68-92.
+// A thread owns one (g,z) column with its M phi_out values in registers and
+// produces DB directions at a time (DB independent FMA chains).
 // ---------------------------------------------------------------------------
 template <int M>
 __global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
                                                     const int32_t *__restrict__ pr_sdoms, int accumulate) {
+  constexpr int DB = 4;
+  extern __shared__ double coef[];
   const DevPR &P = prs[blockIdx.y];
+  stage_coefficients(coef, sd, pr_sdoms, P, M, true);
   const int64_t n = (int64_t)P.ng * P.nz;
   const DevSdom &S0 = sd[pr_sdoms[P.sdom_begin]];
   const bool zfast = S0.fz <= S0.fg;
+  const int nd = S0.nd;
+  const int64_t fa = S0.fa;
+  const gmem<double> *pout = gp(P.phi_out);
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
     int32_t g, z;
     gz_of(e, P.ng, P.nz, zfast, g, z);
     const int64_t po = g * P.mg + z * P.mz;
     double pho[M];
 #pragma unroll
-    for (int nm = 0; nm < M; ++nm) pho[nm] = P.phi_out[po + nm * P.ma];
+    for (int nm = 0; nm < M; ++nm) pho[nm] = pout[po + nm * P.ma];
     for (int k = 0; k < P.sdom_count; ++k) {
-      const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-      double *__restrict__ rhs = S.rhs + g * S.fg + z * S.fz;
-      const double *__restrict__ elp = S.ell_plus;
-      const int64_t fa = S.fa;
-      for (int d = 0; d < S.nd; ++d) {
-        double acc = accumulate ? rhs[d * fa] : 0.0;
+      gmem<double> *rhs = gp(sd[pr_sdoms[P.sdom_begin + k]].rhs) + g * S0.fg + z * S0.fz;
+      const double *elp = coef + k * nd * M;
+      for (int d0 = 0; d0 < nd; d0 += DB) {
+        double acc[DB];
 #pragma unroll
-        for (int nm = 0; nm < M; ++nm) acc = __fma_rn(elp[d * M + nm], pho[nm], acc);
-        __builtin_nontemporal_store(acc, rhs + d * fa);
+        for (int b = 0; b < DB; ++b) acc[b] = (accumulate && d0 + b < nd) ? rhs[(d0 + b) * fa] : 0.0;
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) {
+#pragma unroll
+          for (int b = 0; b < DB; ++b) acc[b] = __fma_rn(elp[((d0 + b) % nd) * M + nm], pho[nm], acc[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < DB; ++b)
+          if (d0 + b < nd) __builtin_nontemporal_store(acc[b], rhs + (d0 + b) * fa);
       }
     }
   }
@@ -307,7 +341,7 @@ int launch_ltimes(const LaunchCtx &L, int accumulate) {
   dim3 grid(grid_for(L.max_gz), L.num_pr);
   switch (L.num_moments) {
 #define X(m) \
-  case m: hipLaunchKernelGGL(k_ltimes<m>, grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
+  case m: hipLaunchKernelGGL(k_ltimes<m>, grid, dim3(256), L.coef_lds_bytes, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
     KR_MOMENT_CASES(X)
 #undef X
     default: set_error("LTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
@@ -320,7 +354,7 @@ int launch_lplustimes(const LaunchCtx &L, int accumulate) {
   dim3 grid(grid_for(L.max_gz), L.num_pr);
   switch (L.num_moments) {
 #define X(m) \
-  case m: hipLaunchKernelGGL(k_lplustimes<m>, grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
+  case m: hipLaunchKernelGGL(k_lplustimes<m>, grid, dim3(256), L.coef_lds_bytes, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
     KR_MOMENT_CASES(X)
 #undef X
     default: set_error("LPlusTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;

@@ -84,12 +84,15 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   const int ni = S.ni, R = U.pg_count;
   const int64_t NU = (int64_t)R * NPJ * NPK * ni;
   const int64_t T = NU + (PJ - 1) + (PK - 1);
-  const double *__restrict__ coefA = S.coef[0];
-  const double *__restrict__ coefB = S.coef[1];
-  const double *__restrict__ coefC = S.coef[2];
-  const double *__restrict__ rhs = S.rhs;
-  const double *__restrict__ sig = S.sig;
-  double *__restrict__ psi = S.psi;
+  const gmem<double> *coefA = gp(S.coef[0]);
+  const gmem<double> *coefB = gp(S.coef[1]);
+  const gmem<double> *coefC = gp(S.coef[2]);
+  const gmem<double> *rhs = gp(S.rhs);
+  const gmem<double> *sig = gp(S.sig);
+  gmem<double> *psi = gp(S.psi);
+  const gmem<double> *in0 = gp(PL.in[0]), *in1 = gp(PL.in[1]), *in2 = gp(PL.in[2]);
+  gmem<double> *out0 = gp(PL.out[0]), *out1 = gp(PL.out[1]), *out2 = gp(PL.out[2]);
+  gmem<double> *own1 = gp(S.plane[1]), *own2 = gp(S.plane[2]);
   const int64_t fz = S.fz, sz = S.sz;
 
   double ring_q[RING], ring_s[RING], ring_a[RING];
@@ -127,19 +130,19 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
             cs = item_state(S, sh, U.pg_begin + cc.r, q, jj, kk, cc);
             bj = coefB[cs.d * S.nj + cs.j];
             ck = coefC[cs.d * S.nk + cs.k];
-            const double *src = PL.in[0];
+            const gmem<double> *src = in0;
             fx = src ? src[cs.d * S.pa[0] + cs.g * S.pg[0] + ((int64_t)cs.k * S.nj + cs.j) * S.pf[0]] : 0.0;
           }
           const int i = S.id > 0 ? cc.ii : ni - 1 - cc.ii;
           double fy, fzv;
           if (jj == 0) {
-            const double *src = (cc.pj == 0) ? PL.in[1] : S.plane[1];
+            const gmem<double> *src = (cc.pj == 0) ? in1 : own1;
             fy = src ? src[cs.d * S.pa[1] + cs.g * S.pg[1] + ((int64_t)cs.k * ni + i) * S.pf[1]] : 0.0;
           } else {
             fy = fyb[prev * nt + tid - Q];
           }
           if (kk == 0) {
-            const double *src = (cc.pk == 0) ? PL.in[2] : S.plane[2];
+            const gmem<double> *src = (cc.pk == 0) ? in2 : own2;
             fzv = src ? src[cs.d * S.pa[2] + cs.g * S.pg[2] + ((int64_t)cs.j * ni + i) * S.pf[2]] : 0.0;
           } else {
             fzv = fzb[prev * nt + tid - Q * PJ];
@@ -153,15 +156,15 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
           fx = p2 - fx;
           const double fyo = p2 - fy, fzo = p2 - fzv;
           if (cc.ii == ni - 1)
-            PL.out[0][cs.d * S.pa[0] + cs.g * S.pg[0] + ((int64_t)cs.k * S.nj + cs.j) * S.pf[0]] = fx;
+            out0[cs.d * S.pa[0] + cs.g * S.pg[0] + ((int64_t)cs.k * S.nj + cs.j) * S.pf[0]] = fx;
           if (jj == PJ - 1) {
-            double *dst = (cc.pj == NPJ - 1) ? PL.out[1] : S.plane[1];
+            gmem<double> *dst = (cc.pj == NPJ - 1) ? out1 : own1;
             dst[cs.d * S.pa[1] + cs.g * S.pg[1] + ((int64_t)cs.k * ni + i) * S.pf[1]] = fyo;
           } else {
             fyb[cur * nt + tid] = fyo;
           }
           if (kk == PK - 1) {
-            double *dst = (cc.pk == NPK - 1) ? PL.out[2] : S.plane[2];
+            gmem<double> *dst = (cc.pk == NPK - 1) ? out2 : own2;
             dst[cs.d * S.pa[2] + cs.g * S.pg[2] + ((int64_t)cs.j * ni + i) * S.pf[2]] = fzo;
           } else {
             fzb[cur * nt + tid] = fzo;
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(256) void k_sweep_coef(const DevSdom *__restrict__ 
       r -= n[ax];
       ++ax;
     }
-    S.coef[ax][d * n[ax] + r] = 2.0 * S.cosines[ax][d] / S.delta[ax][r];
+    gp(S.coef[ax])[d * n[ax] + r] = 2.0 * gp(S.cosines[ax])[d] / gp(S.delta[ax])[r];
   }
 }
 
@@ -222,10 +225,32 @@ static int largest_divisor_le(int n, int cap) {
 SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode) {
   SweepShape sh{};
   sh.ref_mode = ref_mode;
+  sh.nd = nd;
+  sh.ni = ni;
+  sh.nj = nj;
+  sh.nk = nk;
   const bool zfast = (nesting == KR_DGZ || nesting == KR_GDZ);
   // inner pair index = the faster of (d, g) in psi's memory order
   sh.inner_is_g = (nesting == KR_DGZ || nesting == KR_DZG || nesting == KR_ZDG) ? 1 : 0;
   const int inner = sh.inner_is_g ? ng : nd;
+  if (zfast) {
+    // row-scan kernel: a row group of RG >= ni lanes per k-row, all k-rows in one workgroup
+    // (lane l of a row group also holds the x-face of row j = l, so RG >= nj;
+    // the once-per-item faces are prefetched 8 steps early, so nj >= 8)
+    int rg = 8;
+    while (rg < ni || rg < nj) rg *= 2;
+    const size_t lds = (2 * (size_t)rg * nk + (size_t)nd * (ni + nj + nk)) * sizeof(double);
+    if (rg <= 64 && rg * nk <= 1024 && nj >= 8 && lds <= 65536) {
+      sh.kind = 1;
+      sh.RG = rg;
+      sh.Q = 1;
+      sh.PJ = nj;
+      sh.PK = nk;
+      sh.NPJ = 1;
+      sh.NPK = 1;
+      return sh;
+    }
+  }
   if (zfast) {
     sh.Q = 1;
   } else {
@@ -258,6 +283,8 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                  int32_t num_units, const SweepShape &shape) {
   if (num_units <= 0) return KR_OK;
+  if (shape.kind == 1)
+    return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk);
   const int nt = shape.Q * shape.PJ * shape.PK;
   const size_t lds = 4 * (size_t)nt * sizeof(double);
   hipLaunchKernelGGL(k_sweep, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape);
