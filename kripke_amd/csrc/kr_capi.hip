@@ -480,8 +480,13 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   }
 
   // --- sweep schedules --------------------------------------------------------
-  p->shape_ref = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 1);
-  p->shape_pipe = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 0);
+  bool aligned16 = true;
+  for (const DevSdom &H : p->h_sd) {
+    const void *ptrs[] = {H.psi, H.rhs, H.sig, H.plane[0], H.plane[1], H.plane[2]};
+    for (const void *q : ptrs) aligned16 = aligned16 && ((uintptr_t)q % 16 == 0);
+  }
+  p->shape_ref = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 1, aligned16);
+  p->shape_pipe = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 0, aligned16);
   const int32_t pgs = (int32_t)(((int64_t)nd * ng) / p->shape_pipe.Q);
   std::vector<SweepUnit> uref, upipe;
   {

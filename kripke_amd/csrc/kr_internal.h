@@ -111,6 +111,7 @@ struct SweepShape {
   int32_t kind;                 // 0: systolic pencil kernel, 1: row-scan kernel (zone-fastest nestings)
   int32_t Q, PJ, PK, NPJ, NPK;  // lanes per zone column, patch extents, patches per axis
   int32_t RG;                   // row-scan kernel: lanes per row group
+  int32_t V;                    // row-scan kernel: consecutive zones per lane
   int32_t inner_is_g;           // pair index = outer*inner + q ordering
   int32_t ref_mode;             // 1: read/write own planes only (Kernel::sweepSubdomain)
   int32_t nd, ni, nj, nk;       // subdomain extents

@@ -44,6 +44,8 @@ int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *plan
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).
 int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // Shape choice for a subdomain extent and nesting (pair lanes, patch).
-SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode);
+// aligned16: every psi/rhs/sigma_t/plane chunk starts on a 16-byte boundary.
+SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,
+                              bool aligned16);
 
 }  // namespace kr

@@ -22,6 +22,9 @@
 // reference computes them per zone.
 #include "kr_launch.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 namespace kr {
 
 constexpr int RING = 8;  // prefetch depth (steps) of rhs / sigt / a_i
@@ -222,7 +225,38 @@ static int largest_divisor_le(int n, int cap) {
   return 1;
 }
 
-SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode) {
+// Row-scan shape for zone-fastest nestings: V zones per lane (V > 1 needs
+// 16-byte aligned rows), W >= ni/V lanes per row group, one row group per
+// k-row.  Returns false when the subdomain does not fit one workgroup.
+static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, bool aligned16) {
+  int want = ni % 2 == 0 ? 2 : 1;
+  if (const char *e = getenv("KRIPKE_AMD_ROW_ZONES")) want = atoi(e);  // tuning override: 1 or 2
+  const int64_t nz = (int64_t)ni * nj * nk;
+  const int64_t maxface = std::max({(int64_t)nj * nk, (int64_t)ni * nk, (int64_t)ni * nj});
+  // every stream is a buffer resource addressed with 32-bit byte offsets below 2 GiB
+  if ((int64_t)nd * ng * nz * 8 >= (1LL << 31) || (int64_t)nd * ng * maxface * 8 >= (1LL << 31)) return false;
+  for (int V = 2; V >= 1; V /= 2) {
+    if (V > want || ni % V != 0 || (V > 1 && !aligned16)) continue;
+    int W = 4;
+    while (W * V < ni) W *= 2;
+    const size_t lds = (2 * (size_t)W * nk * V + (size_t)nd * (ni + nj + nk)) * sizeof(double);
+    if (W <= 64 && W * nk <= 1024 && lds <= 65536) {
+      sh.kind = 1;
+      sh.RG = W;
+      sh.V = V;
+      sh.Q = 1;
+      sh.PJ = nj;
+      sh.PK = nk;
+      sh.NPJ = 1;
+      sh.NPK = 1;
+      return true;
+    }
+  }
+  return false;
+}
+
+SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,
+                              bool aligned16) {
   SweepShape sh{};
   sh.ref_mode = ref_mode;
   sh.nd = nd;
@@ -233,24 +267,7 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   // inner pair index = the faster of (d, g) in psi's memory order
   sh.inner_is_g = (nesting == KR_DGZ || nesting == KR_DZG || nesting == KR_ZDG) ? 1 : 0;
   const int inner = sh.inner_is_g ? ng : nd;
-  if (zfast) {
-    // row-scan kernel: a row group of RG >= ni lanes per k-row, all k-rows in one workgroup
-    // (lane l of a row group also holds the x-face of row j = l, so RG >= nj;
-    // the once-per-item faces are prefetched 8 steps early, so nj >= 8)
-    int rg = 8;
-    while (rg < ni || rg < nj) rg *= 2;
-    const size_t lds = (2 * (size_t)rg * nk + (size_t)nd * (ni + nj + nk)) * sizeof(double);
-    if (rg <= 64 && rg * nk <= 1024 && nj >= 8 && lds <= 65536) {
-      sh.kind = 1;
-      sh.RG = rg;
-      sh.Q = 1;
-      sh.PJ = nj;
-      sh.PK = nk;
-      sh.NPJ = 1;
-      sh.NPK = 1;
-      return sh;
-    }
-  }
+  if (zfast && row_shape(sh, nd, ng, ni, nj, nk, aligned16)) return sh;
   if (zfast) {
     sh.Q = 1;
   } else {

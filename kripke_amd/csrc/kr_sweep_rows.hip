@@ -1,212 +1,457 @@
 // Diamond-difference sweep for the zone-fastest nestings (DGZ, GDZ), where a
 // zone row (fixed direction, group, j, k; i = 0..ni-1) is contiguous in HBM.
 //
-// A row group of RG lanes (RG = power of two >= ni) owns one k-row of the
-// subdomain and walks its j-rows in sweep order, one row per step; row group
-// kk runs kk steps behind row group 0 (wavefront over (j,k)), and a unit
-// strings several (direction, group) pairs back to back so the fill is paid
-// once.  Within a row the i-recurrence of the incoming x-face,
-//     f_in(i+1) = 2 psi_i - f_in(i),   psi_i = (q_i + a_i f_in(i)) / den_i,
-// is the affine map f -> alpha_i f + beta_i with alpha_i = 2 a_i/den_i - 1,
-// beta_i = 2 q_i/den_i, solved for all lanes at once by an inclusive scan of
-// map compositions (Hillis-Steele, log2 RG shuffle steps).  Every HBM access
-// of rhs, sigma_t and psi is then one contiguous row per row group.
+// Work split.  A row group of W lanes owns one k-row of the subdomain and
+// walks its j-rows in sweep order, one row per step; lane l holds V
+// consecutive zones of the row (W*V >= ni).  Row group kk runs kk steps behind
+// row group 0 (wavefront over (j,k)); a unit strings R (direction, group)
+// items back to back so the pipeline fill is paid once per unit.
 //
-// The j-face stays in a register (same lane, next row); the k-face goes to
-// row group kk+1 through LDS (double-buffered, one barrier per step).  The
-// boundary faces (i-face per row, j-face row at j=0, k-face row at k=0) and
-// rhs / sigma_t rows are prefetched RING steps ahead into registers.
+// Row solve.  Along a row the incoming x-face obeys
+//     f(i+1) = 2 psi_i - f(i),   psi_i = (q_i + a_i f(i)) / den_i,
+// the affine map f -> alpha_i f + beta_i with alpha_i = 2 a_i/den_i - 1 and
+// beta_i = 2 q_i/den_i (q_i: right-hand side plus the y and z face terms).
+// Each lane composes its V maps, the row group scans the lane composites with
+// DPP lane shifts (log2 W steps), and each lane then runs its V zones from the
+// x-face the scan hands it.  The scan re-associates the recurrence: results
+// differ from the sequential order by a few ulps per row.
 //
-// The scan re-associates the x-face recurrence; results differ from the
-// sequential order by a few ulps per row (normwise ~1e-15).
+// Faces.  The j-face stays in registers (same lane, next row); the k-face goes
+// to row group kk+1 through LDS (double-buffered, one barrier per step).  All
+// per-step HBM reads -- rhs, sigma_t, and the incoming x/j/k faces of the row
+// -- are issued D steps ahead into a register ring, unconditionally (lanes
+// that do not need a value read a hot address), so every step issues the
+// same memory instructions and the loads of later steps stay in flight while
+// a step computes.
 #include "kr_launch.h"
 
 namespace kr {
 
 namespace {
 
-constexpr int RRING = 8;
+constexpr int RING = 3;  // steps of prefetch
 
-struct RowCursor {
-  int32_t jj, r;  // row within the item (sweep order), pair within the unit
-};
+typedef double d2 __attribute__((ext_vector_type(2)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 
-__device__ inline void row_advance(RowCursor &c, int nj) {
-  if (++c.jj == nj) {
-    c.jj = 0;
-    ++c.r;
-  }
+// Every HBM stream is a buffer resource (base, byte size): loads and stores
+// take a 32-bit byte offset, and an offset past the end reads zeros / drops
+// the store.  A lane that does not need a value (or a vacuum face, whose
+// resource has size zero) uses NOWHERE, so every step issues the same memory
+// instructions without branches.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t NOWHERE = 0x80000000u;  // resources are smaller than 2 GiB
+constexpr int RSRC_WORD3 = 0x00020000;      // raw 32-bit dword data, no swizzle
+
+__device__ __forceinline__ Rsrc rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_WORD3);
 }
 
-struct RowItem {
-  int32_t d, g;
-  int64_t pbase;  // d*fa + g*fg
-};
+constexpr int NT = 2;  // cache policy: non-temporal (streamed once)
 
-__device__ inline RowItem row_item(const DevSdom &S, int inner_is_g, int pidx) {
-  RowItem it;
-  if (inner_is_g) {
-    it.d = pidx / S.ng;
-    it.g = pidx - it.d * S.ng;
+// V consecutive doubles at byte offset off (16-byte aligned when V = 2).
+template <int V, int AUX>
+__device__ __forceinline__ void load_v(double (&t)[V], Rsrc r, uint32_t off) {
+  if constexpr (V == 1) {
+    t[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
   } else {
-    it.g = pidx / S.nd;
-    it.d = pidx - it.g * S.nd;
+    static_assert(V == 2, "V is 1 or 2");
+    const d2 w = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
+    t[0] = w.x;
+    t[1] = w.y;
   }
-  it.pbase = it.d * S.fa + it.g * S.fg;
-  return it;
 }
 
-// Inclusive scan of affine maps f -> A f + B over the RG lanes of a row
-// group; lane l ends with the composition of maps 0..l (map 0 applied first).
-template <int RG>
-__device__ inline void affine_scan(double &A, double &B, int l) {
+template <int V, int AUX>
+__device__ __forceinline__ void store_v(Rsrc r, uint32_t off, const double (&t)[V]) {
+  if constexpr (V == 1) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, t[0]), r, off, 0, AUX);
+  } else {
+    d2 w;
+    w.x = t[0];
+    w.y = t[1];
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, w), r, off, 0, AUX);
+  }
+}
+
+// Reciprocal: hardware estimate and two Newton steps (error about one ulp).
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __fma_rn(-x, r, 1.0);
+  r = __fma_rn(r, e, r);
+  e = __fma_rn(-x, r, 1.0);
+  return __fma_rn(r, e, r);
+}
+
+// DPP move of a double; lanes without a source lane read zero.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const long long s = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)s, (int)s, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp((int)(s >> 32), (int)(s >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+constexpr int DPP_ROW_SHR = 0x110;  // + shift (1..15), within 16-lane rows
+constexpr int DPP_WAVE_SHR1 = 0x138;
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL>
+__device__ __forceinline__ void compose_from(double &A, double &B) {
+  const double Ap = dpp<CTRL>(A);
+  const double Bp = dpp<CTRL>(B);
+  B = __fma_rn(A, Bp, B);
+  A = A * Ap;
+}
+
+// Inclusive scan of affine maps f -> A f + B over the W lanes of a row group,
+// lane 0 first.  Lane 0's map is constant (A = 0: it already holds the row's
+// incoming face), so every lane whose partial composite reaches back to lane 0
+// has A = 0 and is unchanged by whatever finite map it is composed with next;
+// the shifts therefore need no masking at row-group or row boundaries.
+template <int W>
+__device__ __forceinline__ void scan_maps(double &A, double &B) {
+  if constexpr (W > 1) compose_from<DPP_ROW_SHR + 1>(A, B);
+  if constexpr (W > 2) compose_from<DPP_ROW_SHR + 2>(A, B);
+  if constexpr (W > 4) compose_from<DPP_ROW_SHR + 4>(A, B);
+  if constexpr (W > 8) compose_from<DPP_ROW_SHR + 8>(A, B);
+  if constexpr (W > 16) compose_from<DPP_ROW_BCAST15>(A, B);
+  if constexpr (W > 32) compose_from<DPP_ROW_BCAST31>(A, B);
+}
+
+// Value of the lane one place back in the row group (lane 0: `first`).
+template <int W>
+__device__ __forceinline__ double lane_back(double v, double first, int l) {
+  const double u = W <= 16 ? dpp<DPP_ROW_SHR + 1>(v) : dpp<DPP_WAVE_SHR1>(v);
+  return l == 0 ? first : u;
+}
+
+// Everything one block needs, read once from the tables.  In both
+// zone-fastest nestings the (direction, group) item index p = pg_begin + r
+// gives psi/rhs offset p*nz and face offsets p*nf (nf = faces per plane),
+// since DGZ orders items d*ng + g and GDZ orders them g*nd + d.
+struct RowArgs {
+  const double *rhs, *sig, *coef;
+  double *psi;
+  const double *in[3];
+  double *out[3];
+  int32_t nd, ng, ni, nj, nk, jd, kd;
+  int32_t pg_begin, R, inner_is_g;
+};
+
+// Byte offsets of one row group's current row, stepped with additions.
+// Before the first row the cursor rests on row 0 of the first item; past the
+// last item it keeps the last item, so every offset stays in bounds.
+struct RowCursor {
+  int32_t pos;          // rows since the unit's first row (negative: not started)
+  int32_t jj;           // row within the item, sweep order
+  uint32_t j8, jrow8;   // 8*j, 8*j*ni
+  uint32_t z8;          // 8*(item*nz + lane's zone in row 0)
+  uint32_t s8;          // 8*(g*nz + lane's zone in row 0)
+  uint32_t f8[3];       // x: 8*(item*nf0 + k*nj), j: 8*(item*nf1 + k*ni + lane), k: 8*(item*nf2 + lane)
+  int32_t d, g;
+  int32_t bco;          // d*(ni+nj+nk): the item's coefficient record in LDS
+};
+
+struct RowConst {
+  uint32_t nz8, nf8[3];
+  int32_t L;
+  uint32_t j08, jrow08;            // first row in sweep order
+  int32_t jstep8, jrowstep8;       // per-row increments
+  int32_t rows;                    // R*nj
+};
+
+__device__ __forceinline__ void cursor_init(RowCursor &c, const RowArgs &P, const RowConst &K, int pos,
+                                            uint32_t zk8, uint32_t xk8, uint32_t yk8, uint32_t mb8) {
+  c.pos = pos;
+  c.jj = 0;
+  c.j8 = K.j08;
+  c.jrow8 = K.jrow08;
+  const int p = P.pg_begin;
+  if (P.inner_is_g) {
+    c.d = p / P.ng;
+    c.g = p - c.d * P.ng;
+  } else {
+    c.g = p / P.nd;
+    c.d = p - c.g * P.nd;
+  }
+  c.z8 = (uint32_t)p * K.nz8 + zk8;
+  c.s8 = (uint32_t)c.g * K.nz8 + zk8;
+  c.f8[0] = (uint32_t)p * K.nf8[0] + xk8;
+  c.f8[1] = (uint32_t)p * K.nf8[1] + yk8;
+  c.f8[2] = (uint32_t)p * K.nf8[2] + mb8;
+  c.bco = c.d * K.L;
+}
+
+__device__ __forceinline__ void cursor_step(RowCursor &c, const RowArgs &P, const RowConst &K) {
+  if (++c.pos <= 0) return;
+  if (++c.jj < P.nj) {
+    c.j8 += K.jstep8;
+    c.jrow8 += K.jrowstep8;
+    return;
+  }
+  c.jj = 0;
+  c.j8 = K.j08;
+  c.jrow8 = K.jrow08;
+  if (c.pos >= K.rows) return;  // past the last item: stay on it
+  c.z8 += K.nz8;
 #pragma unroll
-  for (int off = 1; off < RG; off <<= 1) {
-    const double Ap = __shfl_up(A, off, RG);
-    const double Bp = __shfl_up(B, off, RG);
-    if (l >= off) {
-      B = __fma_rn(A, Bp, B);
-      A = A * Ap;
+  for (int x = 0; x < 3; ++x) c.f8[x] += K.nf8[x];
+  if (P.inner_is_g) {
+    if (++c.g == P.ng) {
+      c.g = 0;
+      ++c.d;
+      c.bco += K.L;
+      c.s8 -= (uint32_t)(P.ng - 1) * K.nz8;
+    } else {
+      c.s8 += K.nz8;
+    }
+  } else {
+    if (++c.d == P.nd) {
+      c.d = 0;
+      ++c.g;
+      c.s8 += K.nz8;
+    }
+    c.bco = c.d * K.L;
+  }
+}
+
+template <int SX, int V>
+__device__ __forceinline__ constexpr int memo(int m) {
+  return SX > 0 ? m : V - 1 - m;  // memory slot of sweep position m within a lane's block
+}
+
+template <int W, int V, int SX>
+__device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
+  const int nt = blockDim.x;
+  const int tid = threadIdx.x;
+  const int l = tid % W;
+  const int kk = tid / W;
+  const int ni = P.ni, nj = P.nj, nk = P.nk, nd = P.nd, ng = P.ng;
+  const int PK = nk;
+  const int nl = ni / V;                                // live lanes per row
+  const bool live = l < nl;
+  const int la = live ? l : 0;                          // address lane
+  const int mb = SX > 0 ? la * V : ni - V - la * V;     // first zone (memory order) of the lane's block
+  const int k = P.kd > 0 ? kk : nk - 1 - kk;
+
+  const uint32_t nz = (uint32_t)ni * nj * nk;
+  const uint32_t nf[3] = {(uint32_t)nj * nk, (uint32_t)ni * nk, (uint32_t)ni * nj};
+  RowConst K;
+  K.nz8 = 8 * nz;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) K.nf8[x] = 8 * nf[x];
+  K.L = ni + nj + nk;
+  const int j0 = P.jd > 0 ? 0 : nj - 1;
+  K.j08 = 8 * j0;
+  K.jrow08 = 8 * j0 * ni;
+  K.jstep8 = P.jd > 0 ? 8 : -8;
+  K.jrowstep8 = K.jstep8 * ni;
+  K.rows = P.R * nj;
+
+  // LDS: k-face exchange [2][nt][V], then one coefficient record per
+  // direction: {2cos_x/dx [ni], 2cos_y/dy [nj], 2cos_z/dz [nk]}
+  double *fzb = lds;
+  double *co = lds + 2 * nt * V;
+  {
+    const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], contiguous
+    for (int e = tid; e < nd * K.L; e += nt) {
+      int dst;
+      if (e < nd * ni) {
+        const int d = e / ni;
+        dst = d * K.L + (e - d * ni);
+      } else if (e < nd * (ni + nj)) {
+        const int d = (e - nd * ni) / nj;
+        dst = d * K.L + ni + (e - nd * ni - d * nj);
+      } else {
+        const int d = (e - nd * (ni + nj)) / nk;
+        dst = d * K.L + ni + nj + (e - nd * (ni + nj) - d * nk);
+      }
+      co[dst] = src[e];
+    }
+  }
+  __syncthreads();
+
+  const uint32_t items = (uint32_t)nd * ng;
+  const Rsrc Rrhs = rsrc(P.rhs, items * K.nz8);
+  const Rsrc Rpsi = rsrc(P.psi, items * K.nz8);
+  const Rsrc Rsig = rsrc(P.sig, (uint32_t)ng * K.nz8);
+  Rsrc Rin[3], Rout[3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    Rin[x] = rsrc(P.in[x], P.in[x] ? items * K.nf8[x] : 0u);  // vacuum: reads zeros
+    Rout[x] = rsrc(P.out[x], items * K.nf8[x]);
+  }
+
+  double rq[RING][V], rs[RING][V], rz[RING][V], ry[RING][V], rx[RING];
+
+  RowCursor lc, cc;
+  cursor_init(lc, P, K, -kk, 8u * (uint32_t)(k * nj * ni + mb), 8u * (uint32_t)(k * nj),
+              8u * (uint32_t)(k * ni + mb), 8u * (uint32_t)mb);
+  cc = lc;
+
+  auto refill = [&](int r) {
+    const uint32_t zoff = lc.z8 + lc.jrow8;
+    load_v<V, NT>(rq[r], Rrhs, zoff);
+    load_v<V, 0>(rs[r], Rsig, lc.s8 + lc.jrow8);
+    // faces: the k-face only for row group 0, the j-face only on an item's first row,
+    // the x-face only for lane 0
+    load_v<V, 0>(rz[r], Rin[2], kk == 0 ? lc.f8[2] + lc.jrow8 : NOWHERE);
+    load_v<V, 0>(ry[r], Rin[1], lc.jj == 0 ? lc.f8[1] : NOWHERE);
+    {
+      double t[1];
+      load_v<1, 0>(t, Rin[0], l == 0 ? lc.f8[0] + lc.j8 : NOWHERE);
+      rx[r] = t[0];
+    }
+    cursor_step(lc, P, K);
+  };
+
+#pragma unroll
+  for (int r = 0; r < RING; ++r) refill(r);
+
+  double fy[V];
+#pragma unroll
+  for (int m = 0; m < V; ++m) fy[m] = 0.0;
+  const int T = K.rows + PK - 1;
+
+  for (int t0 = 0; t0 < T; t0 += RING) {
+#pragma unroll
+    for (int r = 0; r < RING; ++r) {
+      const int t = t0 + r;
+      const int cur = t & 1, prev = cur ^ 1;
+      const bool act = cc.pos >= 0 && cc.pos < K.rows;
+      const double bj = co[cc.bco + ni + (int)(cc.j8 >> 3)];
+      const double ck = co[cc.bco + ni + nj + k];
+      const double bc = bj + ck;
+      double a[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) a[m] = co[cc.bco + mb + memo<SX, V>(m)];
+
+      // incoming faces of this row
+      double fz[V];
+      {
+        const int src = kk == 0 ? tid : tid - W;
+#pragma unroll
+        for (int m = 0; m < V; ++m) {
+          const double up = fzb[(prev * nt + src) * V + m];
+          fz[m] = kk == 0 ? rz[r][memo<SX, V>(m)] : up;
+        }
+      }
+      if (cc.jj == 0) {
+#pragma unroll
+        for (int m = 0; m < V; ++m) fy[m] = ry[r][memo<SX, V>(m)];
+      }
+      const double f0 = rx[r];
+
+      // zone maps in sweep order, composed per lane; lane 0 absorbs the row's incoming face
+      double inv[V], q[V];
+      double Ac = 1.0, Bc = 0.0;
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const int e = memo<SX, V>(m);
+        const double den = (a[m] + bc) + rs[r][e];
+        inv[m] = recip(den);
+        q[m] = __fma_rn(fz[m], ck, __fma_rn(fy[m], bj, rq[r][e]));
+        const double i2 = inv[m] + inv[m];
+        const double A = __fma_rn(a[m], i2, -1.0);
+        const double B = q[m] * i2;
+        if (m == 0) {
+          Ac = A;
+          Bc = B;
+        } else {
+          Bc = __fma_rn(A, Bc, B);
+          Ac = A * Ac;
+        }
+      }
+      if (l == 0) {
+        Bc = __fma_rn(Ac, f0, Bc);
+        Ac = 0.0;
+      }
+      scan_maps<W>(Ac, Bc);
+      double f = lane_back<W>(Bc, f0, l);  // x-face entering the lane's first zone
+
+      double p[V], fyo[V], fzo[V];
+#pragma unroll
+      for (int m = 0; m < V; ++m) {
+        const int e = memo<SX, V>(m);
+        p[e] = __fma_rn(a[m], f, q[m]) * inv[m];
+        const double p2 = p[e] + p[e];
+        f = p2 - f;
+        fyo[e] = p2 - fy[m];
+        fzo[e] = p2 - fz[m];
+        fy[m] = fyo[e];
+      }
+
+      const bool st = act && live;
+      store_v<V, NT>(Rpsi, st ? cc.z8 + cc.jrow8 : NOWHERE, p);
+      store_v<V, 0>(Rout[1], (st && cc.jj == nj - 1) ? cc.f8[1] : NOWHERE, fyo);
+      store_v<V, 0>(Rout[2], (st && kk == PK - 1) ? cc.f8[2] + cc.jrow8 : NOWHERE, fzo);
+      {
+        const double t[1] = {f};
+        store_v<1, 0>(Rout[0], (act && l == nl - 1) ? cc.f8[0] + cc.j8 : NOWHERE, t);
+      }
+#pragma unroll
+      for (int m = 0; m < V; ++m) fzb[(cur * nt + tid) * V + m] = fzo[memo<SX, V>(m)];
+      cursor_step(cc, P, K);
+      refill(r);
+      __syncthreads();
     }
   }
 }
 
 }  // namespace
 
-template <int RG>
+template <int W, int V>
 __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__ sd,
                                                      const SweepPlanes *__restrict__ planes,
                                                      const SweepUnit *__restrict__ units, SweepShape sh) {
   extern __shared__ double lds[];
-  const int nt = blockDim.x;
   const SweepUnit U = units[blockIdx.x];
   const DevSdom &S = sd[U.sdom];
   const SweepPlanes &PL = planes[U.sdom];
-  const int tid = threadIdx.x;
-  const int l = tid % RG;   // position along the row in sweep order
-  const int kk = tid / RG;  // k-row (sweep order) owned by this row group
-  const int ni = S.ni, nj = S.nj, nk = S.nk, nd = S.nd;
-  const bool live = l < ni;
-  const int i = live ? (S.id > 0 ? l : ni - 1 - l) : 0;
-  const int k = S.kd > 0 ? kk : nk - 1 - kk;
-  const int R = U.pg_count;
-
-  // LDS: k-face exchange [2][nt], then the 2cos/delta tables A[nd][ni] B[nd][nj] C[nd][nk]
-  double *fzb = lds;
-  double *cA = lds + 2 * nt;
-  double *cB = cA + nd * ni;
-  double *cC = cB + nd * nj;
-  {
-    const gmem<double> *src = gp(S.coef[0]);  // A, B, C are contiguous per subdomain
-    for (int e = tid; e < nd * (ni + nj + nk); e += nt) cA[e] = src[e];
-  }
-  __syncthreads();
-
-  const gmem<double> *rhs = gp(S.rhs);
-  const gmem<double> *sig = gp(S.sig);
-  gmem<double> *psi = gp(S.psi);
-  const gmem<double> *in0 = gp(PL.in[0]), *in1 = gp(PL.in[1]), *in2 = gp(PL.in[2]);
-  gmem<double> *out0 = gp(PL.out[0]), *out1 = gp(PL.out[1]), *out2 = gp(PL.out[2]);
-  const int64_t zrow_k = (int64_t)k * nj;  // z = (k*nj + j)*ni + i
-
-  double rq[RRING], rs[RRING], rz[RRING];
-  // Faces needed once per item, loaded when the loader enters the item (RRING
-  // steps early; nj >= RRING so at most one is pending): the x-faces of all
-  // nj rows (lane l holds row j = l of i_plane) and the j = 0 face row.
-  double xnext = 0.0, ynext = 0.0;
-
-  auto load_slot = [&](int r, const RowCursor &c, const RowItem &it) {
-    const int j = S.jd > 0 ? c.jj : nj - 1 - c.jj;
-    const int64_t z = (zrow_k + j) * ni + i;
-    rq[r] = live ? rhs[it.pbase + z * S.fz] : 0.0;
-    rs[r] = live ? sig[it.g * S.sg + z * S.sz] : 1.0;
-    if (kk == 0) rz[r] = (in2 && live) ? in2[it.d * S.pa[2] + it.g * S.pg[2] + ((int64_t)j * ni + i) * S.pf[2]] : 0.0;
-    if (c.jj == 0) {
-      xnext = (in0 && l < nj) ? in0[it.d * S.pa[0] + it.g * S.pg[0] + (zrow_k + l) * S.pf[0]] : 0.0;
-      ynext = (in1 && live) ? in1[it.d * S.pa[1] + it.g * S.pg[1] + ((int64_t)k * ni + i) * S.pf[1]] : 0.0;
-    }
-  };
-
-  RowCursor lc{-kk, 0};
-  RowItem li = row_item(S, sh.inner_is_g, U.pg_begin);
+  RowArgs P;
+  P.rhs = S.rhs;
+  P.sig = S.sig;
+  P.coef = S.coef[0];
+  P.psi = S.psi;
 #pragma unroll
-  for (int r = 0; r < RRING; ++r) {
-    if (lc.jj >= 0 && lc.r < R) {
-      if (lc.jj == 0) li = row_item(S, sh.inner_is_g, U.pg_begin + lc.r);
-      load_slot(r, lc, li);
-    }
-    row_advance(lc, nj);
+  for (int x = 0; x < 3; ++x) {
+    P.in[x] = PL.in[x];
+    P.out[x] = PL.out[x];
   }
-
-  RowCursor cc{-kk, 0};
-  RowItem ci = row_item(S, sh.inner_is_g, U.pg_begin);
-  double fy = 0.0, a = 0.0, ck = 0.0, xcur = 0.0;
-  const int64_t T = (int64_t)R * nj + (sh.PK - 1);
-
-  for (int64_t t0 = 0; t0 < T; t0 += RRING) {
-#pragma unroll
-    for (int r = 0; r < RRING; ++r) {
-      if (t0 + r < T) {
-        const int cur = r & 1, prev = cur ^ 1;
-        if (cc.jj >= 0 && cc.r < R) {
-          if (cc.jj == 0) {
-            ci = row_item(S, sh.inner_is_g, U.pg_begin + cc.r);
-            a = cA[ci.d * ni + i];
-            ck = cC[ci.d * nk + k];
-            fy = ynext;
-            xcur = xnext;
-          }
-          const int j = S.jd > 0 ? cc.jj : nj - 1 - cc.jj;
-          const double bj = cB[ci.d * nj + j];
-          const double fz = (kk == 0) ? rz[r] : fzb[prev * nt + tid - RG];
-          const double den = ((a + bj) + ck) + rs[r];
-          const double inv = 1.0 / den;
-          const double q = __fma_rn(fz, ck, __fma_rn(fy, bj, rq[r]));
-          double A = live ? __fma_rn(2.0 * a, inv, -1.0) : 1.0;
-          double B = live ? 2.0 * q * inv : 0.0;
-          affine_scan<RG>(A, B, l);
-          const double f0 = __shfl(xcur, j, RG);  // x-face entering row j
-          const double fout = __fma_rn(A, f0, B);
-          double fin = __shfl_up(fout, 1, RG);
-          if (l == 0) fin = f0;
-          const double p = __fma_rn(a, fin, q) * inv;
-          const double p2 = 2.0 * p;
-          const double fyo = p2 - fy, fzo = p2 - fz;
-          fy = fyo;
-          if (live) {
-            const int64_t z = (zrow_k + j) * ni + i;
-            psi[ci.pbase + z * S.fz] = p;
-            if (l == ni - 1) out0[ci.d * S.pa[0] + ci.g * S.pg[0] + (zrow_k + j) * S.pf[0]] = p2 - fin;
-            if (cc.jj == nj - 1) out1[ci.d * S.pa[1] + ci.g * S.pg[1] + ((int64_t)k * ni + i) * S.pf[1]] = fyo;
-            if (kk == sh.PK - 1) out2[ci.d * S.pa[2] + ci.g * S.pg[2] + ((int64_t)j * ni + i) * S.pf[2]] = fzo;
-          }
-          fzb[cur * nt + tid] = fzo;
-        }
-        row_advance(cc, nj);
-        if (lc.jj >= 0 && lc.r < R) {
-          if (lc.jj == 0) li = row_item(S, sh.inner_is_g, U.pg_begin + lc.r);
-          load_slot(r, lc, li);
-        }
-        row_advance(lc, nj);
-        __syncthreads();
-      }
-    }
-  }
+  P.nd = S.nd;
+  P.ng = S.ng;
+  P.ni = S.ni;
+  P.nj = S.nj;
+  P.nk = S.nk;
+  P.jd = S.jd;
+  P.kd = S.kd;
+  P.pg_begin = U.pg_begin;
+  P.R = U.pg_count;
+  P.inner_is_g = sh.inner_is_g;
+  if (S.id > 0)
+    rows_body<W, V, 1>(P, lds);
+  else
+    rows_body<W, V, -1>(P, lds);
 }
 
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                       int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk) {
   if (num_units <= 0) return KR_OK;
-  const int nt = shape.RG * shape.PK;
-  const size_t lds = (2 * (size_t)nt + (size_t)nd * (ni + nj + nk)) * sizeof(double);
-  switch (shape.RG) {
-#define X(rg)                                                                                                  \
-  case rg:                                                                                                     \
-    hipLaunchKernelGGL(k_sweep_rows<rg>, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape);        \
+  const int W = shape.RG, V = shape.V;
+  const int nt = W * nk;
+  const size_t lds = (2 * (size_t)nt * V + (size_t)nd * (ni + nj + nk)) * sizeof(double);
+  switch (W * 8 + V) {
+#define X(w, v)                                                                                              \
+  case w * 8 + v:                                                                                            \
+    hipLaunchKernelGGL((k_sweep_rows<w, v>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape);  \
     break;
-    X(8) X(16) X(32) X(64)
+    X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(64, 1)
+    X(4, 2) X(8, 2) X(16, 2) X(32, 2) X(64, 2)
 #undef X
-    default: set_error("row sweep: unsupported row-group width"); return KR_ERR_UNSUPPORTED;
+    default: set_error("row sweep: unsupported row-group shape"); return KR_ERR_UNSUPPORTED;
   }
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
