@@ -58,7 +58,6 @@ struct kr_problem {
   int64_t sigs_str[4] = {0, 0, 0, 0};
   int32_t *d_sigs_flags = nullptr;
   int64_t max_gz = 0, max_gmix = 0;
-  size_t coef_lds_bytes = 0;
   // timing
   int timing = 0;
   std::vector<TimedCall> pending;
@@ -112,7 +111,7 @@ LaunchCtx launch_ctx(kr_problem *p) {
   L.num_moments = p->M;
   L.max_gz = p->max_gz;
   L.max_gmix = p->max_gmix;
-  L.coef_lds_bytes = p->coef_lds_bytes;
+  L.nd = p->h_sd.empty() ? 1 : p->h_sd[0].nd;
   return L;
 }
 
@@ -431,7 +430,6 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     for (int s = 0; s < p->nsdom; ++s)
       if (p->sdoms[s].pr_chunk == c) pr_sdoms.push_back(s);
     H.sdom_count = (int32_t)pr_sdoms.size() - H.sdom_begin;
-    p->coef_lds_bytes = std::max(p->coef_lds_bytes, (size_t)H.sdom_count * nd * p->M * sizeof(double));
     Str3 m = flux_strides(p->nesting, p->M, ng, nz);
     H.ma = m.a; H.mg = m.g; H.mz = m.z;
     p->pr_gset[c] = S.gset;
@@ -453,8 +451,6 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     H.nz = (int32_t)nz;
     p->max_gmix = std::max<int64_t>(p->max_gmix, (int64_t)ng * R.num_mixelem);
   }
-
-  if (p->coef_lds_bytes > 65536) return bad("directions x moments per PR chunk exceed the 64 KiB LDS stage");
 
   // --- sigs structure -------------------------------------------------------
   {

@@ -24,6 +24,17 @@ __device__ __forceinline__ gmem<T> *gp(T *p) {
   return (gmem<T> *)p;
 }
 
+// Read-only tables that stay constant for a whole launch (quadrature,
+// ell/ell_plus): wave-uniform reads through this address space become scalar
+// loads into SGPRs, which VALU instructions take as operands directly.
+template <class T>
+using cmem = __attribute__((address_space(4))) T;
+
+template <class T>
+__device__ __forceinline__ const cmem<T> *cp(const T *p) {
+  return (const cmem<T> *)p;
+}
+
 // Strides of a (A, Group, Zone) field, A = Direction or Moment, for a
 // nesting (src/Kripke/VarTypes.h:68-101).  Plane fields use the same rule
 // with the face index (slow*nfast + fast) in place of the zone.

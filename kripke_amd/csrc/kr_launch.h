@@ -19,9 +19,9 @@ struct LaunchCtx {
   int64_t sigs_dims[4];     // materials, legendre, groups, groups
   int32_t *sigs_flags;      // device, SIGS_FLAG_BLOCKS ints: nonzero = a coupling entry seen
   int num_pr, num_moments;
+  int nd;                   // directions per subdomain (the same for every subdomain)
   int64_t max_gz;           // max groups*zones over PR chunks
   int64_t max_gmix;         // max groups*mixed elements over PR chunks
-  size_t coef_lds_bytes;    // ell/ell_plus of one PR chunk's subdomains
 };
 
 constexpr int SIGS_FLAG_BLOCKS = 64;

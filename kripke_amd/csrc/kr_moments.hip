@@ -23,111 +23,100 @@ __device__ inline void gz_of(int64_t e, int32_t ng, int32_t nz, bool zfast, int3
   }
 }
 
-// Coefficients of a PR chunk's subdomains staged in LDS: ell or ell_plus of
-// every contributing subdomain, [k][d][nm] (both fields are [d][nm]).
-// Wave-uniform addresses make the reads LDS broadcasts.
-__device__ inline void stage_coefficients(double *lds, const DevSdom *__restrict__ sd,
-                                          const int32_t *__restrict__ pr_sdoms, const DevPR &P, int M, bool plus) {
-  const int nd = sd[pr_sdoms[P.sdom_begin]].nd;
-  const int per = nd * M;
-  for (int k = 0; k < P.sdom_count; ++k) {
-    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    const gmem<double> *src = gp(plus ? S.ell_plus : S.ell);
-    for (int e = threadIdx.x; e < per; e += blockDim.x) lds[k * per + e] = src[e];
-  }
-  __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
 // LTimes: src/Kripke/Kernel/LTimes.cpp:54-65 (body), :79-105 (sdom loop).
 // A thread owns one (g,z) column: it issues the psi loads of a batch of DB
-// directions, then folds them into M accumulators, direction by direction.
+// directions, then folds them into M accumulators, direction by direction,
+// with ell read through the scalar cache (SGPR operands of the FMAs).
 // ---------------------------------------------------------------------------
-template <int M>
+// DB directions of one column: all psi loads first, then the FMAs.
+template <int M, int DB>
+__device__ __forceinline__ void ltimes_batch(double (&acc)[M], const gmem<double> *psi, int64_t fa,
+                                             const cmem<double> *ell) {
+  double p[DB];
+#pragma unroll
+  for (int b = 0; b < DB; ++b) p[b] = __builtin_nontemporal_load(psi + b * fa);
+  __builtin_amdgcn_sched_barrier(0);  // keep all DB loads in flight before the first FMA
+#pragma unroll
+  for (int b = 0; b < DB; ++b) {
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[b * M + nm], p[b], acc[nm]);
+  }
+}
+
+template <int M, int DB, bool ACC>
 __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
-                                                const int32_t *__restrict__ pr_sdoms, int accumulate) {
-  constexpr int DB = 16;
-  extern __shared__ double coef[];
+                                                const int32_t *__restrict__ pr_sdoms) {
   const DevPR &P = prs[blockIdx.y];
-  stage_coefficients(coef, sd, pr_sdoms, P, M, false);
   const int64_t n = (int64_t)P.ng * P.nz;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
   const DevSdom &S0 = sd[pr_sdoms[P.sdom_begin]];
   const bool zfast = S0.fz <= S0.fg;
   const int nd = S0.nd;
   const int64_t fa = S0.fa;
+  int32_t g, z;
+  gz_of(e, P.ng, P.nz, zfast, g, z);
+  const int64_t po = g * P.mg + z * P.mz;
   gmem<double> *phi = gp(P.phi);
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    int32_t g, z;
-    gz_of(e, P.ng, P.nz, zfast, g, z);
-    const int64_t po = g * P.mg + z * P.mz;
-    double acc[M];
+  double acc[M];
 #pragma unroll
-    for (int nm = 0; nm < M; ++nm) acc[nm] = accumulate ? phi[po + nm * P.ma] : 0.0;
-    for (int k = 0; k < P.sdom_count; ++k) {
-      const gmem<double> *psi = gp(sd[pr_sdoms[P.sdom_begin + k]].psi) + g * S0.fg + z * S0.fz;
-      const double *ell = coef + k * nd * M;
-      for (int d0 = 0; d0 < nd; d0 += DB) {
-        double p[DB];
-#pragma unroll
-        for (int b = 0; b < DB; ++b)
-          if (d0 + b < nd) p[b] = __builtin_nontemporal_load(psi + (d0 + b) * fa);
-#pragma unroll
-        for (int b = 0; b < DB; ++b) {
-          if (d0 + b < nd) {
-            const double *eb = ell + (d0 + b) * M;
-#pragma unroll
-            for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(eb[nm], p[b], acc[nm]);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int nm = 0; nm < M; ++nm) phi[po + nm * P.ma] = acc[nm];
+  for (int nm = 0; nm < M; ++nm) acc[nm] = ACC ? phi[po + nm * P.ma] : 0.0;
+  for (int k = 0; k < P.sdom_count; ++k) {
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+    const gmem<double> *psi = gp(S.psi) + g * S0.fg + z * S0.fz;
+    const cmem<double> *ell = cp(S.ell);
+    for (int d0 = 0; d0 < nd; d0 += DB) ltimes_batch<M, DB>(acc, psi + d0 * fa, fa, ell + d0 * M);  // DB divides nd
   }
+#pragma unroll
+  for (int nm = 0; nm < M; ++nm) phi[po + nm * P.ma] = acc[nm];
 }
 
 // ---------------------------------------------------------------------------
 // LPlusTimes: src/Kripke/Kernel/LPlusTimes.cpp:49-60, :68-92.
 // A thread owns one (g,z) column with its M phi_out values in registers and
-// produces DB directions at a time (DB independent FMA chains).
+// produces DB directions at a time (DB independent FMA chains), with
+// ell_plus read through the scalar cache.
 // ---------------------------------------------------------------------------
-template <int M>
+// DB directions of one column, one FMA chain per direction (moments ascending).
+template <int M, int DB, bool ACC>
+__device__ __forceinline__ void lplus_batch(const double (&pho)[M], gmem<double> *rhs, int64_t fa,
+                                            const cmem<double> *elp) {
+  double acc[DB];
+#pragma unroll
+  for (int b = 0; b < DB; ++b) acc[b] = ACC ? rhs[b * fa] : 0.0;
+#pragma unroll
+  for (int b = 0; b < DB; ++b) {
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) acc[b] = __fma_rn(elp[b * M + nm], pho[nm], acc[b]);
+  }
+#pragma unroll
+  for (int b = 0; b < DB; ++b) __builtin_nontemporal_store(acc[b], rhs + b * fa);
+}
+
+template <int M, int DB, bool ACC>
 __global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
-                                                    const int32_t *__restrict__ pr_sdoms, int accumulate) {
-  constexpr int DB = 4;
-  extern __shared__ double coef[];
+                                                    const int32_t *__restrict__ pr_sdoms) {
   const DevPR &P = prs[blockIdx.y];
-  stage_coefficients(coef, sd, pr_sdoms, P, M, true);
   const int64_t n = (int64_t)P.ng * P.nz;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
   const DevSdom &S0 = sd[pr_sdoms[P.sdom_begin]];
   const bool zfast = S0.fz <= S0.fg;
   const int nd = S0.nd;
   const int64_t fa = S0.fa;
+  int32_t g, z;
+  gz_of(e, P.ng, P.nz, zfast, g, z);
+  const int64_t po = g * P.mg + z * P.mz;
   const gmem<double> *pout = gp(P.phi_out);
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    int32_t g, z;
-    gz_of(e, P.ng, P.nz, zfast, g, z);
-    const int64_t po = g * P.mg + z * P.mz;
-    double pho[M];
+  double pho[M];
 #pragma unroll
-    for (int nm = 0; nm < M; ++nm) pho[nm] = pout[po + nm * P.ma];
-    for (int k = 0; k < P.sdom_count; ++k) {
-      gmem<double> *rhs = gp(sd[pr_sdoms[P.sdom_begin + k]].rhs) + g * S0.fg + z * S0.fz;
-      const double *elp = coef + k * nd * M;
-      for (int d0 = 0; d0 < nd; d0 += DB) {
-        double acc[DB];
-#pragma unroll
-        for (int b = 0; b < DB; ++b) acc[b] = (accumulate && d0 + b < nd) ? rhs[(d0 + b) * fa] : 0.0;
-#pragma unroll
-        for (int nm = 0; nm < M; ++nm) {
-#pragma unroll
-          for (int b = 0; b < DB; ++b) acc[b] = __fma_rn(elp[((d0 + b) % nd) * M + nm], pho[nm], acc[b]);
-        }
-#pragma unroll
-        for (int b = 0; b < DB; ++b)
-          if (d0 + b < nd) __builtin_nontemporal_store(acc[b], rhs + (d0 + b) * fa);
-      }
-    }
+  for (int nm = 0; nm < M; ++nm) pho[nm] = pout[po + nm * P.ma];
+  for (int k = 0; k < P.sdom_count; ++k) {
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+    gmem<double> *rhs = gp(S.rhs) + g * S0.fg + z * S0.fz;
+    const cmem<double> *elp = cp(S.ell_plus);
+    for (int d0 = 0; d0 < nd; d0 += DB) lplus_batch<M, DB, ACC>(pho, rhs + d0 * fa, fa, elp + d0 * M);  // DB divides nd
   }
 }
 
@@ -337,11 +326,42 @@ static inline int grid_for(int64_t n, int per_block = 256, int max_blocks = 8192
 
 #define KR_MOMENT_CASES(X) X(1) X(4) X(9) X(16) X(25) X(36) X(49) X(64)
 
+// Directions per batch: the largest of 16, 8, 4, 2, 1 that divides nd.
+static int batch_of(int nd, int cap) {
+  int db = cap;
+  while (db > 1 && nd % db != 0) db /= 2;
+  return db;
+}
+
+template <int M, bool ACC>
+static void ltimes_launch(const LaunchCtx &L, dim3 grid, int db) {
+  switch (db) {
+    case 16: hipLaunchKernelGGL((k_ltimes<M, 16, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 8: hipLaunchKernelGGL((k_ltimes<M, 8, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 4: hipLaunchKernelGGL((k_ltimes<M, 4, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 2: hipLaunchKernelGGL((k_ltimes<M, 2, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    default: hipLaunchKernelGGL((k_ltimes<M, 1, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+  }
+}
+
+template <int M, bool ACC>
+static void lplus_launch(const LaunchCtx &L, dim3 grid, int db) {
+  switch (db) {
+    case 4: hipLaunchKernelGGL((k_lplustimes<M, 4, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 2: hipLaunchKernelGGL((k_lplustimes<M, 2, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    default: hipLaunchKernelGGL((k_lplustimes<M, 1, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+  }
+}
+
 int launch_ltimes(const LaunchCtx &L, int accumulate) {
-  dim3 grid(grid_for(L.max_gz), L.num_pr);
+  dim3 grid(grid_for(L.max_gz, 256, 1 << 30), L.num_pr);
+  const int db = batch_of(L.nd, 16);
   switch (L.num_moments) {
-#define X(m) \
-  case m: hipLaunchKernelGGL(k_ltimes<m>, grid, dim3(256), L.coef_lds_bytes, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
+#define X(m)                                          \
+  case m:                                             \
+    if (accumulate) ltimes_launch<m, true>(L, grid, db); \
+    else ltimes_launch<m, false>(L, grid, db);          \
+    break;
     KR_MOMENT_CASES(X)
 #undef X
     default: set_error("LTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
@@ -351,10 +371,14 @@ int launch_ltimes(const LaunchCtx &L, int accumulate) {
 }
 
 int launch_lplustimes(const LaunchCtx &L, int accumulate) {
-  dim3 grid(grid_for(L.max_gz), L.num_pr);
+  dim3 grid(grid_for(L.max_gz, 256, 1 << 30), L.num_pr);
+  const int db = batch_of(L.nd, 4);
   switch (L.num_moments) {
-#define X(m) \
-  case m: hipLaunchKernelGGL(k_lplustimes<m>, grid, dim3(256), L.coef_lds_bytes, L.stream, L.sd, L.pr, L.pr_sdoms, accumulate); break;
+#define X(m)                                         \
+  case m:                                            \
+    if (accumulate) lplus_launch<m, true>(L, grid, db); \
+    else lplus_launch<m, false>(L, grid, db);          \
+    break;
     KR_MOMENT_CASES(X)
 #undef X
     default: set_error("LPlusTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
