@@ -98,6 +98,7 @@ SIGNATURES = [
     ("kr_source", ctypes.c_int, [P, ctypes.c_double]),
     ("kr_sweep_subdomain", ctypes.c_int, [P, ctypes.c_int32]),
     ("kr_sweep_solver", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    ("kr_sweep_solver_population", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, DP]),
     ("kr_population", ctypes.c_int, [P, DP]),
     ("kr_problem_set_timing", ctypes.c_int, [P, ctypes.c_int32]),
     ("kr_problem_timer", ctypes.c_int, [P, ctypes.c_char_p, DP, ctypes.POINTER(ctypes.c_int64)]),
@@ -200,6 +201,13 @@ class KernelProblem:
     def sweep_solver(self, sdoms):
         arr = (ctypes.c_int32 * len(sdoms))(*sdoms)
         check(lib().kr_sweep_solver(self.h, arr, len(sdoms)), "kr_sweep_solver")
+
+    def sweep_solver_population(self, sdoms):
+        arr = (ctypes.c_int32 * len(sdoms))(*sdoms)
+        v = ctypes.c_double()
+        check(lib().kr_sweep_solver_population(self.h, arr, len(sdoms), ctypes.byref(v)),
+              "kr_sweep_solver_population")
+        return v.value
 
     def population(self):
         v = ctypes.c_double()

@@ -97,7 +97,8 @@ void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bo
 }
 
 // SteadyStateSolver.cpp:44-104.  mode 0: the reference call sequence;
-// mode 1: fills folded into the kernels and the on-device sweep pipeline.
+// mode 1: fills folded into the kernels, the on-device sweep pipeline, and
+// the population summed while the sweep writes psi.
 double Kripke::SteadyStateIteration(DataStore &ds, int mode) {
   auto &pspace = ds.getVariable<PartitionSpace>("pspace");
   const int n = (int)pspace.getNumSubdomains(SPACE_PQR);
@@ -117,7 +118,9 @@ double Kripke::SteadyStateIteration(DataStore &ds, int mode) {
     Kernel::scatteringOverwrite(ds);
     Kernel::source(ds);
     Kernel::LPlusTimesOverwrite(ds);
-    SweepSolver(ds, all, true);
+    double part = 0.0;  // SweepSolver + population in one pass over psi
+    check(kr_sweep_solver_population(kp(ds), all.data(), (int32_t)all.size(), &part), "SweepSolver");
+    return part;
   }
   return Kernel::population(ds);
 }

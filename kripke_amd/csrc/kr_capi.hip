@@ -48,6 +48,8 @@ struct kr_problem {
   std::vector<int32_t> pipe_level_begin, pipe_level_count;
   SweepUnit *d_units_tmp = nullptr;            // scratch for arbitrary lists
   int32_t units_tmp_cap = 0;
+  double *d_pop_units = nullptr;               // fused population: one partial per sweep unit
+  int64_t pop_units_cap = 0;
   // population
   void *d_pop_blocks = nullptr;
   int64_t pop_nblocks = 0;
@@ -537,7 +539,7 @@ int kr_problem_destroy(kr_problem *p) {
   }
   void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
                   p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
-                  p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags};
+                  p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units};
   for (void *b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -588,9 +590,9 @@ int kr_sweep_subdomain(kr_problem *p, int32_t sdom) {
                       p->ref_unit_count[sdom], p->shape_ref);
 }
 
-int kr_sweep_solver(kr_problem *p, const int32_t *list, int32_t num) {
-  if (!p || (!list && num > 0) || num < 0) return fail(KR_ERR_INVALID, "kr_sweep_solver: bad argument");
-  CallTimer t(p, "SweepSolver");
+// SweepSolver body; pop (zone-fastest nestings only) receives one population
+// partial per sweep unit, indexed like the units of the launches.
+static int sweep_solver_impl(kr_problem *p, const int32_t *list, int32_t num, double **pop_units, int64_t *npop) {
   {
     int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, 0, p->nsdom);
     if (rc) return rc;
@@ -598,14 +600,29 @@ int kr_sweep_solver(kr_problem *p, const int32_t *list, int32_t num) {
   bool all = (num == p->nsdom);
   for (int32_t i = 0; all && i < num; ++i) all = (list[i] == i);
   if (all) {
+    double *pop = nullptr;
+    if (pop_units) {
+      const int64_t n = p->pipe_level_begin.empty() ? 0
+                                                    : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
+      if (n > p->pop_units_cap) {
+        if (p->d_pop_units) KR_HIP_CHECK(hipFree(p->d_pop_units));
+        p->d_pop_units = nullptr;
+        KR_HIP_CHECK(hipMalloc((void **)&p->d_pop_units, std::max<int64_t>(n, 1) * sizeof(double)));
+        p->pop_units_cap = n;
+      }
+      pop = p->d_pop_units;
+      *pop_units = pop;
+      *npop = n;
+    }
     for (size_t lv = 0; lv < p->pipe_level_begin.size(); ++lv) {
       CallTimer tk(p, "k_sweep");  // one sweep-kernel launch (one zone-set stage)
       int rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
-                            p->pipe_level_count[lv], p->shape_pipe);
+                            p->pipe_level_count[lv], p->shape_pipe, pop ? pop + p->pipe_level_begin[lv] : nullptr);
       if (rc) return rc;
     }
     return KR_OK;
   }
+  if (pop_units) return fail(KR_ERR_INVALID, "sweep: fused population needs the identity subdomain list");
   std::vector<int32_t> v(list, list + num);
   std::vector<std::vector<int32_t>> levels;
   int rc = sweep_levels(p, v, levels);
@@ -631,6 +648,46 @@ int kr_sweep_solver(kr_problem *p, const int32_t *list, int32_t num) {
     if (rc) return rc;
   }
   return KR_OK;
+}
+
+int kr_sweep_solver(kr_problem *p, const int32_t *list, int32_t num) {
+  if (!p || (!list && num > 0) || num < 0) return fail(KR_ERR_INVALID, "kr_sweep_solver: bad argument");
+  CallTimer t(p, "SweepSolver");
+  return sweep_solver_impl(p, list, num, nullptr, nullptr);
+}
+
+int kr_sweep_solver_population(kr_problem *p, const int32_t *list, int32_t num, double *population) {
+  if (!p || !list || !population || num != p->nsdom)
+    return fail(KR_ERR_INVALID, "kr_sweep_solver_population: the list must name every subdomain once");
+  std::vector<int32_t> seen(p->nsdom, 0);
+  for (int32_t i = 0; i < num; ++i) {
+    if (list[i] < 0 || list[i] >= p->nsdom || seen[list[i]]++)
+      return fail(KR_ERR_INVALID, "kr_sweep_solver_population: the list must name every subdomain once");
+  }
+  // the identity order is the pipeline's own schedule; any other complete
+  // list sweeps the same dependency levels, so it is swept in that order
+  std::vector<int32_t> ident(p->nsdom);
+  for (int32_t s = 0; s < p->nsdom; ++s) ident[s] = s;
+  if (p->shape_pipe.kind == 1) {
+    double *pu = nullptr;
+    int64_t n = 0;
+    {
+      CallTimer t(p, "SweepSolver");
+      int rc = sweep_solver_impl(p, ident.data(), p->nsdom, &pu, &n);
+      if (rc) return rc;
+    }
+    {
+      CallTimer t(p, "Population");
+      int rc = launch_sum(p->ctx->stream, pu, n, p->d_pop_out);
+      if (rc) return rc;
+    }
+    KR_HIP_CHECK(hipMemcpyAsync(population, p->d_pop_out, sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream));
+    KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+    return KR_OK;
+  }
+  int rc = kr_sweep_solver(p, ident.data(), p->nsdom);
+  if (rc) return rc;
+  return kr_population(p, population);
 }
 
 int kr_population(kr_problem *p, double *out) {

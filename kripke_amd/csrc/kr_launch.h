@@ -36,11 +36,15 @@ int launch_fill(hipStream_t st, double *p, int64_t n, double v);
 int launch_copy(hipStream_t st, double *d, const double *s, int64_t n);
 
 // Sweep: one launch sweeps every unit; units of one launch share `shape`.
+// pop (row-scan kernel only): one population partial per unit, or nullptr.
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                 int32_t num_units, const SweepShape &shape);
+                 int32_t num_units, const SweepShape &shape, double *pop = nullptr);
 // Row-scan sweep of the zone-fastest nestings (kr_sweep_rows.hip).
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk);
+                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
+                      double *pop);
+// Sum of n doubles in a fixed order (one workgroup): out[0] = sum(partial[0..n)).
+int launch_sum(hipStream_t st, const double *partial, int64_t n, double *out);
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).
 int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // Shape choice for a subdomain extent and nesting (pair lanes, patch).

@@ -430,6 +430,12 @@ int launch_population(const LaunchCtx &L, const void *blocks, int64_t num_blocks
   return KR_OK;
 }
 
+int launch_sum(hipStream_t st, const double *partial, int64_t n, double *out) {
+  hipLaunchKernelGGL(k_pop_final, dim3(1), dim3(256), 0, st, partial, n, out);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
 int64_t population_block_table(const DevSdom *host_sd, int nsdom, std::vector<char> &bytes) {
   std::vector<PopBlock> b;
   const int64_t per_block = 256 * 16;

@@ -239,7 +239,7 @@ static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_
     if (V > want || ni % V != 0 || (V > 1 && !aligned16)) continue;
     int W = 4;
     while (W * V < ni) W *= 2;
-    const size_t lds = (2 * (size_t)W * nk * V + (size_t)nd * (ni + nj + nk)) * sizeof(double);
+    const size_t lds = (2 * (size_t)W * nk * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
     if (W <= 64 && W * nk <= 1024 && lds <= 65536) {
       sh.kind = 1;
       sh.RG = W;
@@ -298,10 +298,14 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
 }
 
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                 int32_t num_units, const SweepShape &shape) {
+                 int32_t num_units, const SweepShape &shape, double *pop) {
   if (num_units <= 0) return KR_OK;
   if (shape.kind == 1)
-    return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk);
+    return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk, pop);
+  if (pop) {
+    set_error("sweep: the systolic kernel has no fused population");
+    return KR_ERR_UNSUPPORTED;
+  }
   const int nt = shape.Q * shape.PJ * shape.PK;
   const size_t lds = 4 * (size_t)nt * sizeof(double);
   hipLaunchKernelGGL(k_sweep, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape);

@@ -29,7 +29,6 @@ namespace kr {
 
 namespace {
 
-constexpr int RING = 3;  // steps of prefetch
 
 typedef double d2 __attribute__((ext_vector_type(2)));
 typedef unsigned v2u __attribute__((ext_vector_type(2)));
@@ -137,6 +136,8 @@ struct RowArgs {
   double *psi;
   const double *in[3];
   double *out[3];
+  const double *w, *vol;  // quadrature weights, zone volumes (population)
+  double *pop;            // population partial of this block, or nullptr
   int32_t nd, ng, ni, nj, nk, jd, kd;
   int32_t pg_begin, R, inner_is_g;
 };
@@ -223,8 +224,10 @@ __device__ __forceinline__ constexpr int memo(int m) {
   return SX > 0 ? m : V - 1 - m;  // memory slot of sweep position m within a lane's block
 }
 
-template <int W, int V, int SX>
+template <int W, int V, int SX, bool POP>
 __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
+  // steps of prefetch: 3 (2 with the population's extra registers)
+  constexpr int RING = POP ? 2 : 3;
   const int nt = blockDim.x;
   const int tid = threadIdx.x;
   const int l = tid % W;
@@ -243,7 +246,7 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   K.nz8 = 8 * nz;
 #pragma unroll
   for (int x = 0; x < 3; ++x) K.nf8[x] = 8 * nf[x];
-  K.L = ni + nj + nk;
+  K.L = ni + nj + nk + 1;  // record: A[ni], B[nj], C[nk], w
   const int j0 = P.jd > 0 ? 0 : nj - 1;
   K.j08 = 8 * j0;
   K.jrow08 = 8 * j0 * ni;
@@ -252,12 +255,12 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   K.rows = P.R * nj;
 
   // LDS: k-face exchange [2][nt][V], then one coefficient record per
-  // direction: {2cos_x/dx [ni], 2cos_y/dy [nj], 2cos_z/dz [nk]}
+  // direction: {2cos_x/dx [ni], 2cos_y/dy [nj], 2cos_z/dz [nk], w}
   double *fzb = lds;
   double *co = lds + 2 * nt * V;
   {
     const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], contiguous
-    for (int e = tid; e < nd * K.L; e += nt) {
+    for (int e = tid; e < nd * (ni + nj + nk); e += nt) {
       int dst;
       if (e < nd * ni) {
         const int d = e / ni;
@@ -271,6 +274,8 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
       }
       co[dst] = src[e];
     }
+    if (POP)
+      for (int d = tid; d < nd; d += nt) co[d * K.L + K.L - 1] = gp(P.w)[d];
   }
   __syncthreads();
 
@@ -278,6 +283,8 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   const Rsrc Rrhs = rsrc(P.rhs, items * K.nz8);
   const Rsrc Rpsi = rsrc(P.psi, items * K.nz8);
   const Rsrc Rsig = rsrc(P.sig, (uint32_t)ng * K.nz8);
+  const Rsrc Rvol = rsrc(P.vol, POP ? K.nz8 : 0u);
+  const uint32_t zk8 = 8u * (uint32_t)(k * nj * ni + mb);  // lane's zone in row j = 0
   Rsrc Rin[3], Rout[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
@@ -286,9 +293,12 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   }
 
   double rq[RING][V], rs[RING][V], rz[RING][V], ry[RING][V], rx[RING];
+  // zone volumes (population) come from L2 one step ahead; that load is
+  // issued before the ring refill so waiting for it leaves the ring in flight
+  double rv[V];
 
   RowCursor lc, cc;
-  cursor_init(lc, P, K, -kk, 8u * (uint32_t)(k * nj * ni + mb), 8u * (uint32_t)(k * nj),
+  cursor_init(lc, P, K, -kk, zk8, 8u * (uint32_t)(k * nj),
               8u * (uint32_t)(k * ni + mb), 8u * (uint32_t)mb);
   cc = lc;
 
@@ -308,12 +318,14 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
     cursor_step(lc, P, K);
   };
 
+  if (POP) load_v<V, 0>(rv, Rvol, zk8 + cc.jrow8);
 #pragma unroll
   for (int r = 0; r < RING; ++r) refill(r);
 
   double fy[V];
 #pragma unroll
   for (int m = 0; m < V; ++m) fy[m] = 0.0;
+  double pacc = 0.0;  // population: sum of (w*psi)*volume over this lane's zones
   const int T = K.rows + PK - 1;
 
   for (int t0 = 0; t0 < T; t0 += RING) {
@@ -385,6 +397,13 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
       }
 
       const bool st = act && live;
+      if (POP) {
+        const double wd = co[cc.bco + K.L - 1];
+        double t = 0.0;
+#pragma unroll
+        for (int e = 0; e < V; ++e) t += (wd * p[e]) * rv[e];
+        pacc += st ? t : 0.0;
+      }
       store_v<V, NT>(Rpsi, st ? cc.z8 + cc.jrow8 : NOWHERE, p);
       store_v<V, 0>(Rout[1], (st && cc.jj == nj - 1) ? cc.f8[1] : NOWHERE, fyo);
       store_v<V, 0>(Rout[2], (st && kk == PK - 1) ? cc.f8[2] + cc.jrow8 : NOWHERE, fzo);
@@ -395,18 +414,32 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
 #pragma unroll
       for (int m = 0; m < V; ++m) fzb[(cur * nt + tid) * V + m] = fzo[memo<SX, V>(m)];
       cursor_step(cc, P, K);
+      if (POP) load_v<V, 0>(rv, Rvol, zk8 + cc.jrow8);
       refill(r);
       __syncthreads();
     }
+  }
+  if (POP) {
+    // fixed-shape tree over the block: the partial does not depend on timing
+    lds[tid] = pacc;
+    __syncthreads();
+    int half = 1;
+    while (half * 2 < nt) half *= 2;
+    for (; half > 0; half >>= 1) {
+      if (tid < half && tid + half < nt) lds[tid] += lds[tid + half];
+      __syncthreads();
+    }
+    if (tid == 0) P.pop[0] = lds[0];
   }
 }
 
 }  // namespace
 
-template <int W, int V>
+template <int W, int V, bool POP>
 __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__ sd,
                                                      const SweepPlanes *__restrict__ planes,
-                                                     const SweepUnit *__restrict__ units, SweepShape sh) {
+                                                     const SweepUnit *__restrict__ units, SweepShape sh,
+                                                     double *__restrict__ pop) {
   extern __shared__ double lds[];
   const SweepUnit U = units[blockIdx.x];
   const DevSdom &S = sd[U.sdom];
@@ -431,22 +464,31 @@ __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__
   P.pg_begin = U.pg_begin;
   P.R = U.pg_count;
   P.inner_is_g = sh.inner_is_g;
+  P.w = S.w;
+  P.vol = S.volume;
+  P.pop = POP ? pop + blockIdx.x : nullptr;
   if (S.id > 0)
-    rows_body<W, V, 1>(P, lds);
+    rows_body<W, V, 1, POP>(P, lds);
   else
-    rows_body<W, V, -1>(P, lds);
+    rows_body<W, V, -1, POP>(P, lds);
 }
 
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk) {
+                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
+                      double *pop) {
   if (num_units <= 0) return KR_OK;
   const int W = shape.RG, V = shape.V;
   const int nt = W * nk;
-  const size_t lds = (2 * (size_t)nt * V + (size_t)nd * (ni + nj + nk)) * sizeof(double);
+  const size_t lds = (2 * (size_t)nt * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
   switch (W * 8 + V) {
 #define X(w, v)                                                                                              \
   case w * 8 + v:                                                                                            \
-    hipLaunchKernelGGL((k_sweep_rows<w, v>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape);  \
+    if (pop)                                                                                                 \
+      hipLaunchKernelGGL((k_sweep_rows<w, v, true>), dim3(num_units), dim3(nt), lds, st, sd, planes, units,  \
+                         shape, pop);                                                                        \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_sweep_rows<w, v, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, \
+                         shape, pop);                                                                        \
     break;
     X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(64, 1)
     X(4, 2) X(8, 2) X(16, 2) X(32, 2) X(64, 2)

@@ -163,6 +163,33 @@ def test_population(ctx, layout):
     assert g == hp.kernels().population(), "population must be deterministic"
 
 
+@pytest.mark.parametrize("layout", ["DGZ", "GDZ", "DZG"])
+@pytest.mark.parametrize("zones", [(8, 8, 8), (12, 6, 10)])
+def test_sweep_solver_population(ctx, layout, zones):
+    """SweepSolver + population in one call: psi bit-identical to
+    kr_sweep_solver, population equal to kr_population to rounding and to
+    the oracle's compensated sum; deterministic across calls."""
+    kw = kwargs_for(layout, zset=(2, 2, 2), groups=8, zones=zones)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(23)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    k = hp.kernels()
+    sdoms = list(range(hp.num_sdoms))
+    k.sweep_solver(sdoms)
+    psi_a = [hp.get("psi", c) for c in range(hp.num_sdoms)]
+    pop_a = k.population()
+    pop_b = k.sweep_solver_population(sdoms[::-1])  # any complete list
+    for c in range(hp.num_sdoms):
+        assert np.array_equal(hp.get("psi", c), psi_a[c]), "psi chunk %d" % c
+    assert abs(pop_b - pop_a) <= 1e-13 * abs(pop_a)
+    orf.sweep_solver()
+    _, comp = orf.population()
+    assert abs(pop_b - comp) <= TOL_REF * abs(comp)
+    assert pop_b == k.sweep_solver_population(sdoms), "fused population must be deterministic"
+    with pytest.raises(Exception):
+        k.sweep_solver_population(sdoms[:-1])
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_iterations_c0(ctx, layout, mode):
