@@ -23,6 +23,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sweep unknowns/s FP64 (64^3z x128g x128d), % HBM peak; adjoint/forward cost"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+SWEEP_STAGES = 4   # zone-set dependency levels of a 2x2x2 decomposition: one sweep dispatch each
 
 WORKLOAD = dict(zones=(64, 64, 64), groups=128, quad=(8, 16), legendre=4, dset=8, gset=1, zset=(2, 2, 2),
                 geometry=1, seed=20261202)
@@ -40,6 +41,21 @@ def algorithmic_bytes(nd_total, ng, nz, moments, num_sdoms_per_zoneset):
     P = moments * ng * nz
     return {"ltimes": 8 * N + 8 * P, "lplustimes": 8 * P + 8 * N,
             "sweep": 16 * N + 8 * ng * nz * num_sdoms_per_zoneset, "N": N, "P": P}
+
+
+def measured_traffic(layout):
+    """HBM bytes per k_sweep_rows dispatch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), for the nesting it
+    was measured on; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "k_sweep_rows_%s_c1_hbm_traffic.txt" % layout.lower())
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch k_sweep_rows:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def run_gpu(args, layout):
@@ -140,8 +156,10 @@ def main():
                    "unknowns": N, "nesting": args.layout, "zone_sets": [2, 2, 2], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": sweep_achieved / 1e9 if sweep_achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": sweep_achieved / HBM_PEAK if sweep_achieved else None, "traffic": None,
-                     "kernel": "k_sweep", "bytes_per_iteration": ab["sweep"],
+                     "frac": sweep_achieved / HBM_PEAK if sweep_achieved else None,
+                     "traffic": measured_traffic(args.layout),
+                     "kernel": "k_sweep_rows" if args.layout in ("DGZ", "GDZ") else "k_sweep",
+                     "alg_bytes_per_launch": ab["sweep"] / SWEEP_STAGES,
                      "avg_launch_ms": sweep_t / max(sweep_n, 1) * 1e3},
         "metric_path": {"ms_per_iteration": path_t * 1e3, "alg_bytes_per_iteration": path_bytes,
                         "hbm_frac": path_bytes / path_t / HBM_PEAK if path_t > 0 else None,

@@ -9,7 +9,7 @@ import math
 
 import numpy as np
 
-from helpers import canonical
+from helpers import LAYOUT_AXES, canonical
 
 
 def ltimes_numpy(o, layout, M):
@@ -164,3 +164,41 @@ def population_fsum(o, layout):
         t = (w[:, None, None] * psi) * vol[None, None, :]
         terms.append(t.ravel())
     return math.fsum(np.concatenate(terms))
+
+
+def mix_maps(o, r):
+    """Mixed-element maps of zone-set chunk r as integer / float arrays."""
+    return (o.get("zone_to_mixelem", r).astype(int), o.get("zone_to_num_mixelem", r).astype(int),
+            o.get("mixelem_to_material", r).astype(int), o.get("mixelem_to_fraction", r))
+
+
+def numpy_iteration(o, layout, M, m2l, strength=1.0):
+    """One source iteration from the oracle's current psi, in numpy."""
+    S0 = o.sdom(0)
+    ng, nz, nd = S0["ng"], S0["ni"] * S0["nj"] * S0["nk"], S0["nd"]
+    phi = ltimes_numpy(o, layout, M)
+    sd = [o.sdom(s) for s in range(o.num_sdoms)]
+    sigs = o.get("sigs").reshape(3, -1, o.inp.num_groups, o.inp.num_groups)
+    phi_out = {}
+    for S in sd:
+        if S["dset"] != 0:
+            continue
+        z2m, z2n, mat, frac = mix_maps(o, S["r"])
+        sz = np.zeros((nz, sigs.shape[1], ng, ng))
+        G0 = S["glo"]
+        for z in range(nz):
+            for k in range(z2n[z]):
+                sz[z] += frac[z2m[z] + k] * sigs[mat[z2m[z] + k], :, G0:G0 + ng, G0:G0 + ng]
+        po_ = np.einsum("zmgh,mhz->mgz", sz[:, m2l], phi[S["pr"]])
+        for k in range(len(mat)):
+            if mat[k] == 0:
+                po_[0, :, int(o.get("mixelem_to_zone", S["r"])[k])] += strength * frac[k]
+        phi_out[S["pr"]] = po_
+    for c, v in phi_out.items():
+        o.set("phi_out", c, np.transpose(v, [("agz").index(ch) for ch in LAYOUT_AXES[layout]]).ravel())
+    rhs = lplustimes_numpy(o, layout, M)
+    for s, v in rhs.items():
+        o.set("rhs", s, np.transpose(v.reshape(nd, ng, nz),
+                                     [("agz").index(ch) for ch in LAYOUT_AXES[layout]]).ravel())
+    psi = global_sweep_numpy(o, layout, o.inp.nx, o.inp.ny, o.inp.nz)
+    return phi, psi

@@ -13,7 +13,8 @@ import pytest
 from scipy.special import lpmv
 
 import pyoracle as po
-from bruteforce import global_sweep_numpy, lplustimes_numpy, ltimes_numpy, population_fsum
+from bruteforce import (global_sweep_numpy, lplustimes_numpy, ltimes_numpy, mix_maps, numpy_iteration,
+                        population_fsum)
 from helpers import LAYOUT_AXES, canonical, rel_normwise
 
 LAYOUTS = list(LAYOUT_AXES)
@@ -158,11 +159,6 @@ def test_ltimes_lplustimes_vs_einsum(layout, kind):
         assert rel_normwise(got, ref_rhs[s]) < 1e-14
 
 
-def _mix(o, r):
-    return (o.get("zone_to_mixelem", r).astype(int), o.get("zone_to_num_mixelem", r).astype(int),
-            o.get("mixelem_to_material", r).astype(int), o.get("mixelem_to_fraction", r))
-
-
 @pytest.mark.parametrize("layout", ["DGZ", "ZGD", "GDZ"])
 def test_scattering_vs_dense_definition(layout):
     """phi_out(nm,g,z) = sum_gp [sum_mix frac*sigs(mat,n,g,gp)] phi(nm,gp,z)
@@ -186,7 +182,7 @@ def test_scattering_vs_dense_definition(layout):
         S = o.sdom(s)
         chunks.setdefault(S["r"], {})[S["gset"]] = S["pr"]
     for r, by_g in chunks.items():
-        z2m, z2n, mat, frac = _mix(o, r)
+        z2m, z2n, mat, frac = mix_maps(o, r)
         sz = np.zeros((nz, L1, G, G))
         for z in range(nz):
             for k in range(z2n[z]):
@@ -209,7 +205,7 @@ def test_source_closed_form(geometry):
         S = o.sdom(s)
         if S["dset"] != 0:
             continue
-        _, _, mat, frac = _mix(o, S["r"])
+        _, _, mat, frac = mix_maps(o, S["r"])
         want = math.fsum(frac[mat == 0])
         phi_out = canonical(o.get("phi_out", S["pr"]), "DGZ", 4, 3, nz)
         for g in range(3):
@@ -304,38 +300,6 @@ def test_population_vs_fsum():
 # --------------------------------------------------------------------------
 # whole iterations
 # --------------------------------------------------------------------------
-def _numpy_iteration(o, layout, M, m2l, strength=1.0):
-    """One source iteration from the oracle's current psi, in numpy."""
-    S0 = o.sdom(0)
-    ng, nz, nd = S0["ng"], S0["ni"] * S0["nj"] * S0["nk"], S0["nd"]
-    phi = ltimes_numpy(o, layout, M)
-    sd = [o.sdom(s) for s in range(o.num_sdoms)]
-    sigs = o.get("sigs").reshape(3, -1, o.inp.num_groups, o.inp.num_groups)
-    phi_out = {}
-    for S in sd:
-        if S["dset"] != 0:
-            continue
-        z2m, z2n, mat, frac = _mix(o, S["r"])
-        sz = np.zeros((nz, sigs.shape[1], ng, ng))
-        G0 = S["glo"]
-        for z in range(nz):
-            for k in range(z2n[z]):
-                sz[z] += frac[z2m[z] + k] * sigs[mat[z2m[z] + k], :, G0:G0 + ng, G0:G0 + ng]
-        po_ = np.einsum("zmgh,mhz->mgz", sz[:, m2l], phi[S["pr"]])
-        for k in range(len(mat)):
-            if mat[k] == 0:
-                po_[0, :, int(o.get("mixelem_to_zone", S["r"])[k])] += strength * frac[k]
-        phi_out[S["pr"]] = po_
-    for c, v in phi_out.items():
-        o.set("phi_out", c, np.transpose(v, [("agz").index(ch) for ch in LAYOUT_AXES[layout]]).ravel())
-    rhs = lplustimes_numpy(o, layout, M)
-    for s, v in rhs.items():
-        o.set("rhs", s, np.transpose(v.reshape(nd, ng, nz),
-                                     [("agz").index(ch) for ch in LAYOUT_AXES[layout]]).ravel())
-    psi = global_sweep_numpy(o, layout, o.inp.nx, o.inp.ny, o.inp.nz)
-    return phi, psi
-
-
 @pytest.mark.parametrize("layout", ["DGZ", "GZD", "ZDG"])
 def test_full_iterations_vs_numpy(layout):
     kw = dict(layout=layout, zones=(4, 4, 4), zset=(2, 1, 2), gset=2, groups=4, quad=(4, 4), legendre=2)
@@ -348,7 +312,7 @@ def test_full_iterations_vs_numpy(layout):
     nz = S0["ni"] * S0["nj"] * S0["nk"]
     for it in range(3):
         _, comp = o.iterate()
-        phi_np, psi_np = _numpy_iteration(n, layout, M, m2l)
+        phi_np, psi_np = numpy_iteration(n, layout, M, m2l)
         for s, v in psi_np.items():
             n.set("psi", s, np.transpose(v, [("agz").index(ch) for ch in LAYOUT_AXES[layout]]).ravel())
         assert comp == pytest.approx(population_fsum(n, layout), rel=1e-12)
