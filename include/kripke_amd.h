@@ -143,9 +143,8 @@ int kr_sweep_subdomain(kr_problem *prob, int32_t sdom_id);
 int kr_sweep_solver(kr_problem *prob, const int32_t *sdom_list, int32_t num);
 /* SweepSolver followed by Kernel::population, the two calls the reference
  * makes back to back (src/Kripke/SteadyStateSolver.cpp:92-99).  The list
- * must name every subdomain once.  On the zone-fastest nestings the
- * population is summed by the sweep kernel as it writes psi (no second pass
- * over psi); otherwise it runs kr_population after the sweep.  Same psi as
+ * must name every subdomain once.  The population is summed by the sweep
+ * kernels as they write psi (no second pass over psi).  Same psi as
  * kr_sweep_solver; the population agrees with kr_population to rounding
  * (deterministic, fixed-order sums).  Blocks until the value is on the host. */
 int kr_sweep_solver_population(kr_problem *prob, const int32_t *sdom_list, int32_t num, double *population);

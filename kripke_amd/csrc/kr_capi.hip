@@ -114,6 +114,7 @@ LaunchCtx launch_ctx(kr_problem *p) {
   L.max_gz = p->max_gz;
   L.max_gmix = p->max_gmix;
   L.nd = p->h_sd.empty() ? 1 : p->h_sd[0].nd;
+  L.dfast = (p->nesting == KR_GZD || p->nesting == KR_ZGD) ? 1 : 0;
   return L;
 }
 
@@ -155,6 +156,31 @@ int flush_timers(kr_problem *p) {
 // Units for the pair groups of one subdomain, R pair groups each.
 void append_units(std::vector<SweepUnit> &u, int sdom, int32_t pgs, int32_t R) {
   for (int32_t b = 0; b < pgs; b += R) u.push_back(SweepUnit{sdom, b, std::min(R, pgs - b), 0});
+}
+
+// The sweep kernels address each unit's slice of psi/rhs, the planes and
+// sigma_t through buffer resources with 32-bit byte offsets: every unit's span
+// (from its first pair's first element to its last pair's last element) must
+// stay below 2 GiB.  Offsets grow with the pair index in every nesting.
+int check_unit_spans(const kr_problem *p, const std::vector<SweepUnit> &units, const SweepShape &sh) {
+  const int64_t lim = 0x7FFFFFF8LL / 8;
+  for (const SweepUnit &u : units) {
+    const DevSdom &S = p->h_sd[u.sdom];
+    const int64_t first = (int64_t)u.pg_begin * sh.Q, last = (int64_t)(u.pg_begin + u.pg_count) * sh.Q - 1;
+    auto off = [&](int64_t pidx, int64_t sa, int64_t sg) {
+      const int64_t d = sh.inner_is_g ? pidx / S.ng : pidx % S.nd;
+      const int64_t g = sh.inner_is_g ? pidx % S.ng : pidx / S.nd;
+      return d * sa + g * sg;
+    };
+    if (off(last, S.fa, S.fg) - off(first, S.fa, S.fg) + (S.nz - 1) * S.fz + 1 > lim)
+      return fail(KR_ERR_UNSUPPORTED, "sweep: a work unit's psi slice exceeds 2 GiB");
+    const int64_t nf[3] = {(int64_t)S.nj * S.nk, (int64_t)S.ni * S.nk, (int64_t)S.ni * S.nj};
+    for (int x = 0; x < 3; ++x)
+      if (off(last, S.pa[x], S.pg[x]) - off(first, S.pa[x], S.pg[x]) + (nf[x] - 1) * S.pf[x] + 1 > lim)
+        return fail(KR_ERR_UNSUPPORTED, "sweep: a work unit's plane slice exceeds 2 GiB");
+    if ((int64_t)S.ng * S.nz > lim) return fail(KR_ERR_UNSUPPORTED, "sweep: sigma_t chunk exceeds 2 GiB");
+  }
+  return KR_OK;
 }
 
 int32_t pick_R(int64_t total_pgs, const SweepShape &sh) {
@@ -506,6 +532,12 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     }
   }
 
+  if (p->shape_ref.kind == 2 || p->shape_pipe.kind == 2)
+    return bad("sweep: directions per subdomain x (ni+nj+nk) exceed the 64 KiB LDS coefficient table");
+  if ((rc = check_unit_spans(p, uref, p->shape_ref)) || (rc = check_unit_spans(p, upipe, p->shape_pipe))) {
+    kr_problem_destroy(p);
+    return rc;
+  }
   std::vector<DevSdom> hsd = p->h_sd;
   std::vector<char> popb;
   p->pop_nblocks = population_block_table(p->h_sd.data(), p->nsdom, popb);
@@ -636,6 +668,7 @@ static int sweep_solver_impl(kr_problem *p, const int32_t *list, int32_t num, do
     for (int32_t s : lv) append_units(u, s, pgs, Rl);
     lc.push_back((int32_t)u.size() - lb.back());
   }
+  if ((rc = check_unit_spans(p, u, p->shape_pipe))) return rc;
   KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
   if ((int32_t)u.size() > p->units_tmp_cap) {
     if (p->d_units_tmp) KR_HIP_CHECK(hipFree(p->d_units_tmp));
@@ -668,7 +701,7 @@ int kr_sweep_solver_population(kr_problem *p, const int32_t *list, int32_t num, 
   // list sweeps the same dependency levels, so it is swept in that order
   std::vector<int32_t> ident(p->nsdom);
   for (int32_t s = 0; s < p->nsdom; ++s) ident[s] = s;
-  if (p->shape_pipe.kind == 1) {
+  {
     double *pu = nullptr;
     int64_t n = 0;
     {
@@ -685,9 +718,6 @@ int kr_sweep_solver_population(kr_problem *p, const int32_t *list, int32_t num, 
     KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
     return KR_OK;
   }
-  int rc = kr_sweep_solver(p, ident.data(), p->nsdom);
-  if (rc) return rc;
-  return kr_population(p, population);
 }
 
 int kr_population(kr_problem *p, double *out) {

@@ -20,6 +20,7 @@ struct LaunchCtx {
   int32_t *sigs_flags;      // device, SIGS_FLAG_BLOCKS ints: nonzero = a coupling entry seen
   int num_pr, num_moments;
   int nd;                   // directions per subdomain (the same for every subdomain)
+  int dfast;                // direction-fastest nesting (GZD, ZGD): psi [column][d], phi [column][nm]
   int64_t max_gz;           // max groups*zones over PR chunks
   int64_t max_gmix;         // max groups*mixed elements over PR chunks
 };

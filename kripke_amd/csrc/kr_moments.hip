@@ -9,6 +9,8 @@
 // LPlusTimes) with fused multiply-adds.
 #include "kr_launch.h"
 
+#include <algorithm>
+
 namespace kr {
 
 // Map a flat (group, zone) index to (g, z) so that consecutive threads are
@@ -117,6 +119,141 @@ __global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ 
     gmem<double> *rhs = gp(S.rhs) + g * S0.fg + z * S0.fz;
     const cmem<double> *elp = cp(S.ell_plus);
     for (int d0 = 0; d0 < nd; d0 += DB) lplus_batch<M, DB, ACC>(pho, rhs + d0 * fa, fa, elp + d0 * M);  // DB divides nd
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Direction-fastest nestings (GZD, ZGD): psi is [column][d] and phi is
+// [column][nm] with the same column order (g*nz+z for GZD, z*ng+g for ZGD),
+// so a thread-per-column mapping would touch one cache line per lane.  A
+// block of CB columns moves its [CB][nd] and [CB][M] slabs through an LDS
+// tile with coalesced loads and stores; each thread then runs its column from
+// LDS with the same FMA order as the column kernels above.
+// ---------------------------------------------------------------------------
+constexpr int DF_CB = 128;  // columns per block
+
+// Slab [n][w] (row-major, n <= DF_CB rows) <-> tile rows of stride ts.
+// Element e = tid + i*DF_CB maps to (e / w, e % w), stepped without division.
+struct SlabWalk {
+  int col, k, dq, dr;
+  __device__ SlabWalk(int tid, int w) : col(tid / w), k(tid % w), dq(DF_CB / w), dr(DF_CB % w) {}
+  __device__ void next(int w) {
+    col += dq;
+    k += dr;
+    if (k >= w) {
+      k -= w;
+      ++col;
+    }
+  }
+};
+
+template <bool NT>
+__device__ __forceinline__ void slab_to_tile(double *tile, int ts, const gmem<double> *src, int total, int w) {
+  constexpr int B = 8;  // loads in flight per thread before the LDS writes
+  SlabWalk sw(threadIdx.x, w);
+  for (int e0 = threadIdx.x; e0 < total; e0 += B * DF_CB) {
+    double v[B];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int e = e0 + b * DF_CB;
+      v[b] = e < total ? (NT ? __builtin_nontemporal_load(src + e) : src[e]) : 0.0;
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      if (e0 + b * DF_CB < total) tile[sw.col * ts + sw.k] = v[b];
+      sw.next(w);
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_to_slab(gmem<double> *dst, const double *tile, int ts, int total, int w) {
+  SlabWalk sw(threadIdx.x, w);
+  for (int e = threadIdx.x; e < total; e += DF_CB) {
+    __builtin_nontemporal_store(tile[sw.col * ts + sw.k], dst + e);
+    sw.next(w);
+  }
+}
+
+template <int M, bool ACC>
+__global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
+                                                        const int32_t *__restrict__ pr_sdoms, int ts) {
+  extern __shared__ double tile[];
+  const DevPR &P = prs[blockIdx.y];
+  const int64_t C = (int64_t)P.ng * P.nz;
+  const int64_t c0 = (int64_t)blockIdx.x * DF_CB;
+  if (c0 >= C) return;
+  const int cn = (int)min<int64_t>(DF_CB, C - c0);
+  const int tid = threadIdx.x;
+  const int nd = sd[pr_sdoms[P.sdom_begin]].nd;
+  gmem<double> *phi = gp(P.phi) + c0 * M;
+  double acc[M];
+  if (ACC) {
+    slab_to_tile<false>(tile, ts, phi, cn * M, M);
+    __syncthreads();
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) acc[nm] = tid < cn ? tile[tid * ts + nm] : 0.0;
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) acc[nm] = 0.0;
+  }
+  for (int k = 0; k < P.sdom_count; ++k) {
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+    slab_to_tile<true>(tile, ts, gp(S.psi) + c0 * nd, cn * nd, nd);
+    __syncthreads();
+    const cmem<double> *ell = cp(S.ell);
+    if (tid < cn) {
+      for (int d = 0; d < nd; ++d) {
+        const double p = tile[tid * ts + d];
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[d * M + nm], p, acc[nm]);
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int nm = 0; nm < M; ++nm) tile[tid * ts + nm] = acc[nm];
+  __syncthreads();
+  tile_to_slab(phi, tile, ts, cn * M, M);
+}
+
+template <int M, bool ACC>
+__global__ __launch_bounds__(DF_CB) void k_lplustimes_dfast(const DevSdom *__restrict__ sd,
+                                                            const DevPR *__restrict__ prs,
+                                                            const int32_t *__restrict__ pr_sdoms, int ts) {
+  extern __shared__ double tile[];
+  const DevPR &P = prs[blockIdx.y];
+  const int64_t C = (int64_t)P.ng * P.nz;
+  const int64_t c0 = (int64_t)blockIdx.x * DF_CB;
+  if (c0 >= C) return;
+  const int cn = (int)min<int64_t>(DF_CB, C - c0);
+  const int tid = threadIdx.x;
+  const int nd = sd[pr_sdoms[P.sdom_begin]].nd;
+  slab_to_tile<false>(tile, ts, gp(P.phi_out) + c0 * M, cn * M, M);
+  __syncthreads();
+  double pho[M];
+#pragma unroll
+  for (int nm = 0; nm < M; ++nm) pho[nm] = tid < cn ? tile[tid * ts + nm] : 0.0;
+  __syncthreads();
+  for (int k = 0; k < P.sdom_count; ++k) {
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+    gmem<double> *rhs = gp(S.rhs) + c0 * nd;
+    if (ACC) {
+      slab_to_tile<false>(tile, ts, rhs, cn * nd, nd);
+      __syncthreads();
+    }
+    const cmem<double> *elp = cp(S.ell_plus);
+    if (tid < cn) {
+      for (int d = 0; d < nd; ++d) {
+        double acc = ACC ? tile[tid * ts + d] : 0.0;
+#pragma unroll
+        for (int nm = 0; nm < M; ++nm) acc = __fma_rn(elp[d * M + nm], pho[nm], acc);
+        tile[tid * ts + d] = acc;
+      }
+    }
+    __syncthreads();
+    tile_to_slab(rhs, tile, ts, cn * nd, nd);
+    __syncthreads();
   }
 }
 
@@ -333,8 +470,21 @@ static int batch_of(int nd, int cap) {
   return db;
 }
 
+// LDS tile row stride for the direction-fastest kernels: odd, so that a
+// thread-per-row read of consecutive lanes hits distinct banks.  Tiles above
+// 64 KiB (very many directions or moments) fall back to the column kernels.
+static int dfast_stride(int nd, int M) { return std::max(nd, M) | 1; }
+static bool dfast_fits(int nd, int M) { return (size_t)DF_CB * dfast_stride(nd, M) * sizeof(double) <= 65536; }
+
 template <int M, bool ACC>
 static void ltimes_launch(const LaunchCtx &L, dim3 grid, int db) {
+  if (L.dfast && dfast_fits(L.nd, M)) {
+    const int ts = dfast_stride(L.nd, M);
+    dim3 g((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
+    hipLaunchKernelGGL((k_ltimes_dfast<M, ACC>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double), L.stream,
+                       L.sd, L.pr, L.pr_sdoms, ts);
+    return;
+  }
   switch (db) {
     case 16: hipLaunchKernelGGL((k_ltimes<M, 16, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
     case 8: hipLaunchKernelGGL((k_ltimes<M, 8, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
@@ -346,6 +496,13 @@ static void ltimes_launch(const LaunchCtx &L, dim3 grid, int db) {
 
 template <int M, bool ACC>
 static void lplus_launch(const LaunchCtx &L, dim3 grid, int db) {
+  if (L.dfast && dfast_fits(L.nd, M)) {
+    const int ts = dfast_stride(L.nd, M);
+    dim3 g((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
+    hipLaunchKernelGGL((k_lplustimes_dfast<M, ACC>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double), L.stream,
+                       L.sd, L.pr, L.pr_sdoms, ts);
+    return;
+  }
   switch (db) {
     case 4: hipLaunchKernelGGL((k_lplustimes<M, 4, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
     case 2: hipLaunchKernelGGL((k_lplustimes<M, 2, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;

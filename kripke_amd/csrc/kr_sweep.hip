@@ -1,25 +1,35 @@
 // Diamond-difference sweep (src/Kripke/Kernel/SweepSubdomain.cpp:26-110)
-// as a systolic pencil wavefront on gfx950.
+// as a systolic pencil wavefront on gfx950, for the nestings whose fastest
+// index is a group or a direction (DZG, ZDG, GZD, ZGD).
 //
 // A workgroup owns a patch of (j,k) zone columns of one subdomain, times Q
-// adjacent (direction, group) pairs.  Each thread walks its i-pencil one
-// zone per step; the thread at (jj,kk) runs (jj+kk) steps behind the patch
-// corner, so zone (i,j,k) is computed at step i+j+k after its upwind
-// neighbours (i-1,j,k), (i,j-1,k), (i,j,k-1).  The i-face stays in a
-// register, the j- and k-faces pass to the neighbouring thread through LDS
-// (double-buffered, one barrier per step).  A unit strings several pair
-// groups and patches back to back through the same pipeline, so the
-// wavefront fill is paid once per unit.  Faces that leave a patch go to the
-// subdomain's own plane chunk (the reference's running-plane storage);
-// faces that leave the subdomain go to the downwind subdomain's plane chunk
-// (pipeline mode) or stay in the own chunk (Kernel::sweepSubdomain mode).
+// adjacent (direction, group) pairs (adjacent in memory, so the Q lanes of a
+// zone read one contiguous run).  Each thread walks its i-pencil one zone per
+// step; the thread at (jj,kk) runs (jj+kk) steps behind the patch corner, so
+// zone (i,j,k) is computed after its upwind neighbours (i-1,j,k), (i,j-1,k),
+// (i,j,k-1).  The i-face stays in a register, the j- and k-faces pass to the
+// neighbouring thread through LDS (double-buffered, one barrier per step).  A
+// unit strings several pair groups and patches back to back through the same
+// pipeline, so the wavefront fill is paid once per unit.  Faces that leave a
+// patch go to the subdomain's own plane chunk (the reference's running-plane
+// storage) and are read back by the next patch; faces that leave the
+// subdomain go to the downwind subdomain's plane chunk (pipeline mode) or
+// stay in the own chunk (Kernel::sweepSubdomain mode).
+//
+// Memory pipeline: every stream is a buffer resource; rhs, sigma_t and the
+// three incoming faces of a step are read RING steps ahead into a register
+// ring, unconditionally (a thread that needs no face reads NOWHERE, which
+// returns zero without a memory access), so the wait before a step's use
+// never drains the loads of later steps.  The 2cos/delta tables and the
+// quadrature weights sit in LDS.  A patch-boundary face is written at least
+// RING + 1 steps (and so at least one barrier) before the next patch's
+// prefetch reads it; the shape chooser enforces that distance.
 //
 // Arithmetic per zone (bit-identical across nestings and schedules):
 //   num = fma(fz, c_k, fma(fy, b_j, fma(fx, a_i, rhs)))
 //   psi = num / (((a_i + b_j) + c_k) + sigt)          (IEEE division)
 //   f_out = 2*psi - f_in                              (exact scaling)
-// with a_i = 2*xcos(d)/dx(i) etc. precomputed on the host exactly as the
-// reference computes them per zone.
+// with a_i = 2*xcos(d)/dx(i) etc. as the reference computes them per zone.
 #include "kr_launch.h"
 
 #include <algorithm>
@@ -27,169 +37,312 @@
 
 namespace kr {
 
-constexpr int RING = 8;  // prefetch depth (steps) of rhs / sigt / a_i
+namespace {
 
-struct Cursor {
-  int32_t ii, pj, pk, r;
+constexpr int COL_RING = 4;  // steps of prefetch
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr uint32_t NOWHERE = 0x80000000u;  // resources are smaller than 2 GiB
+constexpr int RSRC_WORD3 = 0x00020000;      // raw 32-bit dword data, no swizzle
+constexpr int NT = 2;                       // cache policy: non-temporal
+
+__device__ __forceinline__ Rsrc rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, RSRC_WORD3);
+}
+template <int AUX>
+__device__ __forceinline__ double ld(Rsrc r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
+}
+template <int AUX>
+__device__ __forceinline__ void st(Rsrc r, uint32_t off, double v) {
+  typedef unsigned v2u __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, off, 0, AUX);
+}
+
+struct ColArgs {
+  const double *rhs, *sig, *coef, *w, *vol;
+  double *psi;
+  const double *own[3];   // the subdomain's plane chunks
+  double *out[3];         // destination of faces leaving the subdomain
+  int32_t has_in[3];      // 0: vacuum face (reads as zero)
+  double *pop;
+  int64_t fa, fg, fz, sg, sz;
+  int64_t pa[3], pg[3], pf[3];
+  int64_t base_f, base_p[3];  // element offset of the unit's first pair in psi/rhs and in the planes
+  int32_t nd, ng, ni, nj, nk, jd, kd;
+  int32_t pg_begin, R, inner_is_g;
 };
 
-__device__ inline void advance(Cursor &c, int ni, int NPJ, int NPK) {
-  if (++c.ii == ni) {
-    c.ii = 0;
-    if (++c.pj == NPJ) {
-      c.pj = 0;
-      if (++c.pk == NPK) {
-        c.pk = 0;
+// Position of one thread in its unit: u = ((r*NPK + pk)*NPJ + pj)*ni + ii,
+// and the byte offsets of its current zone.  Before its first step a thread
+// rests on u = 0; past the last pair group it keeps cycling the last one, so
+// every offset stays in bounds.
+struct ColCursor {
+  int32_t pos;               // steps since this thread's start (negative: not started)
+  int32_t ii, pj, pk, r;
+  int32_t d, g, i, j, k;
+  uint32_t psi8, sig8;       // 8*(d*fa + g*fg + z*fz), 8*(g*sg + z*sz)
+  uint32_t x8, y8, z8;       // face offsets: i-plane row (k,j); j-plane (k,i); k-plane (j,i)
+  uint32_t vol8;             // 8*z
+};
+
+struct ColConst {
+  int Q, PJ, PK, NPJ, NPK;
+  int q, jj, kk;
+  int istep;                 // +1 / -1: zone i per step
+  int32_t dpsi8, dsig8, dy8, dz8, dvol8;  // per-step offset increments
+};
+
+__device__ __forceinline__ void col_pair(ColCursor &c, const ColArgs &P, const ColConst &K) {
+  const int pidx = (P.pg_begin + min(c.r, P.R - 1)) * K.Q + K.q;
+  if (P.inner_is_g) {
+    c.d = pidx / P.ng;
+    c.g = pidx - c.d * P.ng;
+  } else {
+    c.g = pidx / P.nd;
+    c.d = pidx - c.g * P.nd;
+  }
+}
+
+// Offsets of the first zone of the current patch row (ii = 0).
+__device__ __forceinline__ void col_patch(ColCursor &c, const ColArgs &P, const ColConst &K) {
+  const int jl = c.pj * K.PJ + K.jj, kl = c.pk * K.PK + K.kk;
+  c.j = P.jd > 0 ? jl : P.nj - 1 - jl;
+  c.k = P.kd > 0 ? kl : P.nk - 1 - kl;
+  c.i = K.istep > 0 ? 0 : P.ni - 1;
+  const int64_t z = ((int64_t)c.k * P.nj + c.j) * P.ni + c.i;
+  c.psi8 = (uint32_t)(8 * (c.d * P.fa + c.g * P.fg - P.base_f + z * P.fz));
+  c.sig8 = (uint32_t)(8 * (c.g * P.sg + z * P.sz));
+  c.x8 = (uint32_t)(8 * (c.d * P.pa[0] + c.g * P.pg[0] - P.base_p[0] + ((int64_t)c.k * P.nj + c.j) * P.pf[0]));
+  c.y8 = (uint32_t)(8 * (c.d * P.pa[1] + c.g * P.pg[1] - P.base_p[1] + ((int64_t)c.k * P.ni + c.i) * P.pf[1]));
+  c.z8 = (uint32_t)(8 * (c.d * P.pa[2] + c.g * P.pg[2] - P.base_p[2] + ((int64_t)c.j * P.ni + c.i) * P.pf[2]));
+  c.vol8 = (uint32_t)(8 * z);
+}
+
+__device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const ColConst &K, int pos) {
+  c.pos = pos;
+  c.ii = c.pj = c.pk = c.r = 0;
+  col_pair(c, P, K);
+  col_patch(c, P, K);
+}
+
+__device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const ColConst &K) {
+  if (++c.pos <= 0) return;
+  if (++c.ii < P.ni) {
+    c.i += K.istep;
+    c.psi8 += K.dpsi8;
+    c.sig8 += K.dsig8;
+    c.y8 += K.dy8;
+    c.z8 += K.dz8;
+    c.vol8 += K.dvol8;
+    return;
+  }
+  c.ii = 0;
+  if (++c.pj == K.NPJ) {
+    c.pj = 0;
+    if (++c.pk == K.NPK) {
+      c.pk = 0;
+      if (c.r + 1 < P.R) {  // past the last pair group: stay on it
         ++c.r;
+        col_pair(c, P, K);
       }
     }
   }
+  col_patch(c, P, K);
 }
 
-struct ItemState {
-  int32_t d, g, j, k;
-  int64_t fbase, sbase;
-};
-
-__device__ inline ItemState item_state(const DevSdom &S, const SweepShape &sh, int pg, int q, int jj, int kk,
-                                       const Cursor &c) {
-  ItemState st;
-  const int pidx = pg * sh.Q + q;
-  if (sh.inner_is_g) {
-    st.d = pidx / S.ng;
-    st.g = pidx - st.d * S.ng;
-  } else {
-    st.g = pidx / S.nd;
-    st.d = pidx - st.g * S.nd;
-  }
-  const int jl = c.pj * sh.PJ + jj, kl = c.pk * sh.PK + kk;
-  st.j = S.jd > 0 ? jl : S.nj - 1 - jl;
-  st.k = S.kd > 0 ? kl : S.nk - 1 - kl;
-  const int64_t zrow = ((int64_t)st.k * S.nj + st.j) * S.ni;
-  st.fbase = st.d * S.fa + st.g * S.fg + zrow * S.fz;
-  st.sbase = st.g * S.sg + zrow * S.sz;
-  return st;
-}
-
-__global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes,
-                                                const SweepUnit *__restrict__ units, SweepShape sh) {
-  extern __shared__ double lds[];
+template <bool POP>
+__device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, double *lds) {
+  constexpr int RING = COL_RING;
   const int nt = blockDim.x;
-  double *fyb = lds;           // [2][nt]
-  double *fzb = lds + 2 * nt;  // [2][nt]
+  const int tid = threadIdx.x;
+  const int ni = P.ni, nj = P.nj, nk = P.nk, nd = P.nd, ng = P.ng;
+  const int L = ni + nj + nk + 1;  // coefficient record per direction: A[ni], B[nj], C[nk], w
+  double *fyb = lds;               // [2][nt]
+  double *fzb = lds + 2 * nt;      // [2][nt]
+  double *co = lds + 4 * nt;       // [nd][L]
+  {
+    const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], contiguous
+    for (int e = tid; e < nd * (ni + nj + nk); e += nt) {
+      int dst;
+      if (e < nd * ni) {
+        const int d = e / ni;
+        dst = d * L + (e - d * ni);
+      } else if (e < nd * (ni + nj)) {
+        const int d = (e - nd * ni) / nj;
+        dst = d * L + ni + (e - nd * ni - d * nj);
+      } else {
+        const int d = (e - nd * (ni + nj)) / nk;
+        dst = d * L + ni + nj + (e - nd * (ni + nj) - d * nk);
+      }
+      co[dst] = src[e];
+    }
+    if (POP)
+      for (int d = tid; d < nd; d += nt) co[d * L + L - 1] = gp(P.w)[d];
+  }
+  __syncthreads();
+
+  // resources start at the unit's first pair; the host keeps each unit's
+  // span below 2 GiB (unit_spans_fit), so 32-bit offsets cover it
+  const int64_t items = (int64_t)nd * ng;
+  const uint32_t nz = (uint32_t)ni * nj * nk;
+  const int64_t nf[3] = {(int64_t)nj * nk, (int64_t)ni * nk, (int64_t)ni * nj};
+  auto span = [](int64_t elems) { return (uint32_t)std::min<int64_t>(elems * 8, 0x7FFFFFF8LL); };
+  const Rsrc Rrhs = rsrc(P.rhs + P.base_f, span(items * nz - P.base_f));
+  const Rsrc Rpsi = rsrc(P.psi + P.base_f, span(items * nz - P.base_f));
+  const Rsrc Rsig = rsrc(P.sig, span((int64_t)ng * nz));
+  const Rsrc Rvol = rsrc(P.vol, POP ? nz * 8 : 0u);
+  const Rsrc Rin0 = rsrc(P.own[0] + P.base_p[0], P.has_in[0] ? span(items * nf[0] - P.base_p[0]) : 0u);
+  const Rsrc Rown1 = rsrc(P.own[1] + P.base_p[1], span(items * nf[1] - P.base_p[1]));
+  const Rsrc Rown2 = rsrc(P.own[2] + P.base_p[2], span(items * nf[2] - P.base_p[2]));
+  const Rsrc Rout0 = rsrc(P.out[0] + P.base_p[0], span(items * nf[0] - P.base_p[0]));
+  const Rsrc Rout1 = rsrc(P.out[1] + P.base_p[1], span(items * nf[1] - P.base_p[1]));
+  const Rsrc Rout2 = rsrc(P.out[2] + P.base_p[2], span(items * nf[2] - P.base_p[2]));
+
+  double rq[RING], rs[RING], rx[RING], ry[RING], rz[RING];
+  ColCursor lc, cc;
+  col_init(lc, P, K, -(K.jj + K.kk));
+  cc = lc;
+
+  auto refill = [&](int r) {
+    rq[r] = ld<NT>(Rrhs, lc.psi8);
+    rs[r] = ld<0>(Rsig, lc.sig8);
+    // incoming faces: x at a pencil start; y for the first j-row of a patch
+    // (the subdomain's incoming face on the first patch, else the previous
+    // patch's outgoing face in the own chunk); z likewise
+    rx[r] = ld<0>(Rin0, lc.ii == 0 ? lc.x8 : NOWHERE);
+    ry[r] = ld<0>(Rown1, (K.jj == 0 && (lc.pj > 0 || P.has_in[1])) ? lc.y8 : NOWHERE);
+    rz[r] = ld<0>(Rown2, (K.kk == 0 && (lc.pk > 0 || P.has_in[2])) ? lc.z8 : NOWHERE);
+    col_step(lc, P, K);
+  };
+
+  double rv = 0.0;
+  if (POP) rv = ld<0>(Rvol, cc.vol8);
+#pragma unroll
+  for (int r = 0; r < RING; ++r) refill(r);
+
+  const int rows = P.R * K.NPK * K.NPJ * ni;
+  const int T = rows + (K.PJ - 1) + (K.PK - 1);
+  double fx = 0.0, pacc = 0.0;
+
+  for (int t0 = 0; t0 < T; t0 += RING) {
+#pragma unroll
+    for (int r = 0; r < RING; ++r) {
+      const int cur = (t0 + r) & 1, prev = cur ^ 1;
+      const bool act = cc.pos >= 0 && cc.pos < rows;
+      const double *rec = co + cc.d * L;
+      const double a = rec[cc.i];
+      const double bj = rec[ni + cc.j];
+      const double ck = rec[ni + nj + cc.k];
+      if (cc.ii == 0) fx = rx[r];
+      const double fy = K.jj == 0 ? ry[r] : fyb[prev * nt + tid - K.Q];
+      const double fzv = K.kk == 0 ? rz[r] : fzb[prev * nt + tid - K.Q * K.PJ];
+      const double num = __fma_rn(fzv, ck, __fma_rn(fy, bj, __fma_rn(fx, a, rq[r])));
+      const double den = ((a + bj) + ck) + rs[r];
+      const double p = num / den;
+      const double p2 = 2.0 * p;
+      fx = p2 - fx;
+      const double fyo = p2 - fy, fzo = p2 - fzv;
+      st<NT>(Rpsi, act ? cc.psi8 : NOWHERE, p);
+      st<0>(Rout0, (act && cc.ii == ni - 1) ? cc.x8 : NOWHERE, fx);
+      const bool yedge = act && K.jj == K.PJ - 1, zedge = act && K.kk == K.PK - 1;
+      const bool ylast = cc.pj == K.NPJ - 1, zlast = cc.pk == K.NPK - 1;
+      st<0>(Rout1, (yedge && ylast) ? cc.y8 : NOWHERE, fyo);
+      st<0>(Rown1, (yedge && !ylast) ? cc.y8 : NOWHERE, fyo);
+      st<0>(Rout2, (zedge && zlast) ? cc.z8 : NOWHERE, fzo);
+      st<0>(Rown2, (zedge && !zlast) ? cc.z8 : NOWHERE, fzo);
+      fyb[cur * nt + tid] = fyo;
+      fzb[cur * nt + tid] = fzo;
+      if (POP) {
+        const double t = (rec[L - 1] * p) * rv;
+        pacc += act ? t : 0.0;
+      }
+      col_step(cc, P, K);
+      if (POP) rv = ld<0>(Rvol, cc.vol8);
+      refill(r);
+      __syncthreads();
+    }
+  }
+  if (POP) {
+    lds[tid] = pacc;  // the face buffers are free after the last barrier
+    __syncthreads();
+    int half = 1;
+    while (half * 2 < nt) half *= 2;
+    for (; half > 0; half >>= 1) {
+      if (tid < half && tid + half < nt) lds[tid] += lds[tid + half];
+      __syncthreads();
+    }
+    if (tid == 0) P.pop[0] = lds[0];
+  }
+}
+
+}  // namespace
+
+template <bool POP>
+__global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes,
+                                                const SweepUnit *__restrict__ units, SweepShape sh,
+                                                double *__restrict__ pop) {
+  extern __shared__ double lds[];
   const SweepUnit U = units[blockIdx.x];
   const DevSdom &S = sd[U.sdom];
   const SweepPlanes &PL = planes[U.sdom];
+  ColArgs P;
+  P.rhs = S.rhs;
+  P.sig = S.sig;
+  P.coef = S.coef[0];
+  P.w = S.w;
+  P.vol = S.volume;
+  P.psi = S.psi;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    P.own[x] = S.plane[x];
+    P.out[x] = PL.out[x];
+    P.has_in[x] = PL.in[x] != nullptr;
+    P.pa[x] = S.pa[x];
+    P.pg[x] = S.pg[x];
+    P.pf[x] = S.pf[x];
+  }
+  P.pop = POP ? pop + blockIdx.x : nullptr;
+  P.fa = S.fa;
+  P.fg = S.fg;
+  P.fz = S.fz;
+  P.sg = S.sg;
+  P.sz = S.sz;
+  P.nd = S.nd;
+  P.ng = S.ng;
+  P.ni = S.ni;
+  P.nj = S.nj;
+  P.nk = S.nk;
+  P.jd = S.jd;
+  P.kd = S.kd;
+  P.pg_begin = U.pg_begin;
+  P.R = U.pg_count;
+  P.inner_is_g = sh.inner_is_g;
+  {
+    const int pidx0 = U.pg_begin * sh.Q;  // the unit's first pair: smallest offsets in every field
+    const int d0 = sh.inner_is_g ? pidx0 / S.ng : pidx0 % S.nd;
+    const int g0 = sh.inner_is_g ? pidx0 % S.ng : pidx0 / S.nd;
+    P.base_f = d0 * S.fa + g0 * S.fg;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) P.base_p[x] = d0 * S.pa[x] + g0 * S.pg[x];
+  }
+  ColConst K;
+  K.Q = sh.Q;
+  K.PJ = sh.PJ;
+  K.PK = sh.PK;
+  K.NPJ = sh.NPJ;
+  K.NPK = sh.NPK;
   const int tid = threadIdx.x;
-  const int Q = sh.Q, PJ = sh.PJ, PK = sh.PK, NPJ = sh.NPJ, NPK = sh.NPK;
-  const int q = tid % Q, jj = (tid / Q) % PJ, kk = tid / (Q * PJ);
-  const int s = jj + kk;
-  const int ni = S.ni, R = U.pg_count;
-  const int64_t NU = (int64_t)R * NPJ * NPK * ni;
-  const int64_t T = NU + (PJ - 1) + (PK - 1);
-  const gmem<double> *coefA = gp(S.coef[0]);
-  const gmem<double> *coefB = gp(S.coef[1]);
-  const gmem<double> *coefC = gp(S.coef[2]);
-  const gmem<double> *rhs = gp(S.rhs);
-  const gmem<double> *sig = gp(S.sig);
-  gmem<double> *psi = gp(S.psi);
-  const gmem<double> *in0 = gp(PL.in[0]), *in1 = gp(PL.in[1]), *in2 = gp(PL.in[2]);
-  gmem<double> *out0 = gp(PL.out[0]), *out1 = gp(PL.out[1]), *out2 = gp(PL.out[2]);
-  gmem<double> *own1 = gp(S.plane[1]), *own2 = gp(S.plane[2]);
-  const int64_t fz = S.fz, sz = S.sz;
-
-  double ring_q[RING], ring_s[RING], ring_a[RING];
-
-  // Loader cursor starts at local time u = -s and runs RING steps ahead.
-  Cursor lc{-s, 0, 0, 0};
-  ItemState ls{};
-  bool ls_valid = false;
-#pragma unroll
-  for (int r = 0; r < RING; ++r) {
-    if (lc.ii >= 0 && lc.r < R) {
-      if (!ls_valid || lc.ii == 0) {
-        ls = item_state(S, sh, U.pg_begin + lc.r, q, jj, kk, lc);
-        ls_valid = true;
-      }
-      const int i = S.id > 0 ? lc.ii : ni - 1 - lc.ii;
-      ring_q[r] = rhs[ls.fbase + i * fz];
-      ring_s[r] = sig[ls.sbase + i * sz];
-      ring_a[r] = coefA[ls.d * ni + i];
-    }
-    advance(lc, ni, NPJ, NPK);
-  }
-
-  Cursor cc{-s, 0, 0, 0};
-  ItemState cs{};
-  double fx = 0.0, bj = 0.0, ck = 0.0;
-
-  for (int64_t t0 = 0; t0 < T; t0 += RING) {
-#pragma unroll
-    for (int r = 0; r < RING; ++r) {
-      if (t0 + r < T) {
-        const int cur = r & 1, prev = cur ^ 1;
-        if (cc.ii >= 0 && cc.r < R) {
-          if (cc.ii == 0) {
-            cs = item_state(S, sh, U.pg_begin + cc.r, q, jj, kk, cc);
-            bj = coefB[cs.d * S.nj + cs.j];
-            ck = coefC[cs.d * S.nk + cs.k];
-            const gmem<double> *src = in0;
-            fx = src ? src[cs.d * S.pa[0] + cs.g * S.pg[0] + ((int64_t)cs.k * S.nj + cs.j) * S.pf[0]] : 0.0;
-          }
-          const int i = S.id > 0 ? cc.ii : ni - 1 - cc.ii;
-          double fy, fzv;
-          if (jj == 0) {
-            const gmem<double> *src = (cc.pj == 0) ? in1 : own1;
-            fy = src ? src[cs.d * S.pa[1] + cs.g * S.pg[1] + ((int64_t)cs.k * ni + i) * S.pf[1]] : 0.0;
-          } else {
-            fy = fyb[prev * nt + tid - Q];
-          }
-          if (kk == 0) {
-            const gmem<double> *src = (cc.pk == 0) ? in2 : own2;
-            fzv = src ? src[cs.d * S.pa[2] + cs.g * S.pg[2] + ((int64_t)cs.j * ni + i) * S.pf[2]] : 0.0;
-          } else {
-            fzv = fzb[prev * nt + tid - Q * PJ];
-          }
-          const double a = ring_a[r];
-          const double num = __fma_rn(fzv, ck, __fma_rn(fy, bj, __fma_rn(fx, a, ring_q[r])));
-          const double den = ((a + bj) + ck) + ring_s[r];
-          const double p = num / den;
-          psi[cs.fbase + i * fz] = p;
-          const double p2 = 2.0 * p;
-          fx = p2 - fx;
-          const double fyo = p2 - fy, fzo = p2 - fzv;
-          if (cc.ii == ni - 1)
-            out0[cs.d * S.pa[0] + cs.g * S.pg[0] + ((int64_t)cs.k * S.nj + cs.j) * S.pf[0]] = fx;
-          if (jj == PJ - 1) {
-            gmem<double> *dst = (cc.pj == NPJ - 1) ? out1 : own1;
-            dst[cs.d * S.pa[1] + cs.g * S.pg[1] + ((int64_t)cs.k * ni + i) * S.pf[1]] = fyo;
-          } else {
-            fyb[cur * nt + tid] = fyo;
-          }
-          if (kk == PK - 1) {
-            gmem<double> *dst = (cc.pk == NPK - 1) ? out2 : own2;
-            dst[cs.d * S.pa[2] + cs.g * S.pg[2] + ((int64_t)cs.j * ni + i) * S.pf[2]] = fzo;
-          } else {
-            fzb[cur * nt + tid] = fzo;
-          }
-        }
-        advance(cc, ni, NPJ, NPK);
-        // refill this ring slot with the values for step t + RING
-        if (lc.ii >= 0 && lc.r < R) {
-          if (!ls_valid || lc.ii == 0) {
-            ls = item_state(S, sh, U.pg_begin + lc.r, q, jj, kk, lc);
-            ls_valid = true;
-          }
-          const int i = S.id > 0 ? lc.ii : ni - 1 - lc.ii;
-          ring_q[r] = rhs[ls.fbase + i * fz];
-          ring_s[r] = sig[ls.sbase + i * sz];
-          ring_a[r] = coefA[ls.d * ni + i];
-        }
-        advance(lc, ni, NPJ, NPK);
-        __syncthreads();
-      }
-    }
-  }
+  K.q = tid % sh.Q;
+  K.jj = (tid / sh.Q) % sh.PJ;
+  K.kk = tid / (sh.Q * sh.PJ);
+  K.istep = S.id > 0 ? 1 : -1;
+  K.dpsi8 = (int32_t)(8 * K.istep * S.fz);
+  K.dsig8 = (int32_t)(8 * K.istep * S.sz);
+  K.dy8 = (int32_t)(8 * K.istep * S.pf[1]);
+  K.dz8 = (int32_t)(8 * K.istep * S.pf[2]);
+  K.dvol8 = 8 * K.istep;
+  cols_body<POP>(P, K, lds);
 }
 
 // 2*cos(d)/delta per (direction, zone index), the exact expression of
@@ -233,8 +386,10 @@ static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_
   if (const char *e = getenv("KRIPKE_AMD_ROW_ZONES")) want = atoi(e);  // tuning override: 1 or 2
   const int64_t nz = (int64_t)ni * nj * nk;
   const int64_t maxface = std::max({(int64_t)nj * nk, (int64_t)ni * nk, (int64_t)ni * nj});
-  // every stream is a buffer resource addressed with 32-bit byte offsets below 2 GiB
-  if ((int64_t)nd * ng * nz * 8 >= (1LL << 31) || (int64_t)nd * ng * maxface * 8 >= (1LL << 31)) return false;
+  // every stream is a buffer resource based at the unit's first item and
+  // addressed with 32-bit byte offsets below 2 GiB (units hold <= 64 items)
+  if (64 * nz * 8 >= (1LL << 31) || 64 * maxface * 8 >= (1LL << 31) || (int64_t)ng * nz * 8 >= (1LL << 31))
+    return false;
   for (int V = 2; V >= 1; V /= 2) {
     if (V > want || ni % V != 0 || (V > 1 && !aligned16)) continue;
     int W = 4;
@@ -275,25 +430,34 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     while (qv < 16 && inner % (qv * 2) == 0) qv *= 2;
     sh.Q = qv;
   }
-  const int cap = 1024 / sh.Q;
-  // patch: as many (j,k) columns as fit, subject to the pipeline distances
-  // (a patch's downstream neighbour must start after its face is written).
-  int best_pj = 1, best_pk = 1;
-  for (int pj = largest_divisor_le(nj, cap); pj >= 1; pj = largest_divisor_le(nj, pj - 1)) {
-    const int pk = largest_divisor_le(nk, cap / pj);
-    const int npj = nj / pj;
-    const bool ok_j = (npj == 1) || (ni >= pj);
-    const bool ok_k = (nk / pk == 1) || ((int64_t)npj * ni >= pk);
-    if (ok_j && ok_k && pj * pk > best_pj * best_pk) {
-      best_pj = pj;
-      best_pk = pk;
+  // LDS: j/k face exchange [4][threads] and the coefficient records [nd][ni+nj+nk+1]
+  const size_t coef_lds = (size_t)nd * (ni + nj + nk + 1) * sizeof(double);
+  for (int cap = 1024 / sh.Q; cap >= 1; cap /= 2) {
+    // patch: as many (j,k) columns as fit, subject to the pipeline distances
+    // (a patch's downstream neighbour prefetches a face COL_RING steps before
+    // it uses it, and must do so at least one step after the face is written).
+    int best_pj = 0, best_pk = 0;
+    for (int pj = largest_divisor_le(nj, cap); pj >= 1; pj = largest_divisor_le(nj, pj - 1)) {
+      const int pk = largest_divisor_le(nk, cap / pj);
+      const int npj = nj / pj;
+      const bool ok_j = (npj == 1) || (ni >= pj + COL_RING);
+      const bool ok_k = (nk / pk == 1) || ((int64_t)npj * ni >= pk + COL_RING);
+      if (ok_j && ok_k && pj * pk > best_pj * best_pk) {
+        best_pj = pj;
+        best_pk = pk;
+      }
+      if (pj == 1) break;
     }
-    if (pj == 1) break;
+    if (best_pj == 0) continue;
+    const size_t lds = coef_lds + 4 * (size_t)sh.Q * best_pj * best_pk * sizeof(double);
+    if (lds > 65536) continue;
+    sh.PJ = best_pj;
+    sh.PK = best_pk;
+    sh.NPJ = nj / best_pj;
+    sh.NPK = nk / best_pk;
+    return sh;
   }
-  sh.PJ = best_pj;
-  sh.PK = best_pk;
-  sh.NPJ = nj / best_pj;
-  sh.NPK = nk / best_pk;
+  sh.kind = 2;  // no shape fits: directions x zone extents too large for the LDS coefficient table
   return sh;
 }
 
@@ -302,13 +466,12 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
   if (num_units <= 0) return KR_OK;
   if (shape.kind == 1)
     return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk, pop);
-  if (pop) {
-    set_error("sweep: the systolic kernel has no fused population");
-    return KR_ERR_UNSUPPORTED;
-  }
   const int nt = shape.Q * shape.PJ * shape.PK;
-  const size_t lds = 4 * (size_t)nt * sizeof(double);
-  hipLaunchKernelGGL(k_sweep, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape);
+  const size_t lds = (4 * (size_t)nt + (size_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1)) * sizeof(double);
+  if (pop)
+    hipLaunchKernelGGL(k_sweep<true>, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
+  else
+    hipLaunchKernelGGL(k_sweep<false>, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }

@@ -178,11 +178,13 @@ __device__ __forceinline__ void cursor_init(RowCursor &c, const RowArgs &P, cons
     c.g = p / P.nd;
     c.d = p - c.g * P.nd;
   }
-  c.z8 = (uint32_t)p * K.nz8 + zk8;
+  // psi/rhs and plane offsets are relative to the unit's first item (the
+  // resources are based there), so they stay below 2 GiB for any chunk size
+  c.z8 = zk8;
   c.s8 = (uint32_t)c.g * K.nz8 + zk8;
-  c.f8[0] = (uint32_t)p * K.nf8[0] + xk8;
-  c.f8[1] = (uint32_t)p * K.nf8[1] + yk8;
-  c.f8[2] = (uint32_t)p * K.nf8[2] + mb8;
+  c.f8[0] = xk8;
+  c.f8[1] = yk8;
+  c.f8[2] = mb8;
   c.bco = c.d * K.L;
 }
 
@@ -279,17 +281,20 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   }
   __syncthreads();
 
-  const uint32_t items = (uint32_t)nd * ng;
-  const Rsrc Rrhs = rsrc(P.rhs, items * K.nz8);
-  const Rsrc Rpsi = rsrc(P.psi, items * K.nz8);
+  // resources start at the unit's first item and cover its R items
+  const int64_t p0 = P.pg_begin;
+  const uint32_t span_z = (uint32_t)P.R * K.nz8;
+  const Rsrc Rrhs = rsrc(P.rhs + p0 * nz, span_z);
+  const Rsrc Rpsi = rsrc(P.psi + p0 * nz, span_z);
   const Rsrc Rsig = rsrc(P.sig, (uint32_t)ng * K.nz8);
   const Rsrc Rvol = rsrc(P.vol, POP ? K.nz8 : 0u);
   const uint32_t zk8 = 8u * (uint32_t)(k * nj * ni + mb);  // lane's zone in row j = 0
   Rsrc Rin[3], Rout[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
-    Rin[x] = rsrc(P.in[x], P.in[x] ? items * K.nf8[x] : 0u);  // vacuum: reads zeros
-    Rout[x] = rsrc(P.out[x], items * K.nf8[x]);
+    const uint32_t span_f = (uint32_t)P.R * K.nf8[x];
+    Rin[x] = rsrc(P.in[x] ? P.in[x] + p0 * nf[x] : nullptr, P.in[x] ? span_f : 0u);  // vacuum: reads zeros
+    Rout[x] = rsrc(P.out[x] + p0 * nf[x], span_f);
   }
 
   double rq[RING][V], rs[RING][V], rz[RING][V], ry[RING][V], rx[RING];
