@@ -153,6 +153,39 @@ int kr_sweep_solver_population(kr_problem *prob, const int32_t *sdom_list, int32
  * Blocks until the value is on the host. */
 int kr_population(kr_problem *prob, double *out);
 
+/* ---- transposed operators and parameter gradients ----------------------- */
+/* No counterpart in the reference: the differentiable form of the source
+ * iteration.  Each adjoint keeps its adjoint variable in the field of the
+ * primal variable it belongs to (psibar in psi, rhsbar in rhs, ...), so an
+ * adjoint problem registered with its own psi/rhs/phi/phi_out chunks (and
+ * the forward problem's geometry, materials and quadrature) runs the chain
+ * backwards with no copies:
+ *   kr_ltimes_adjoint          psi      = L^T phi                (overwrite)
+ *   kr_lplustimes_adjoint      phi_out  = (L+)^T rhs             (overwrite)
+ *   kr_scattering_adjoint      phi     (+)= S^T phi_out
+ *   kr_sweep_solver_adjoint    rhs      = H^-T psi  (all subdomains, the
+ *                              transposed diamond-difference sweep: every
+ *                              axis reversed, faces handed to the forward-
+ *                              upwind subdomain, vacuum = zero adjoint inflow)
+ * Per-zone isotropic group-diagonal scattering s(g,z), one array per PR
+ * chunk in sigma_t's (sigt_zonal) layout, for the per-zone sigma_s parameter:
+ *   kr_scattering_zonal        phi_out(0,g,z) (+)= s(g,z) phi(0,g,z), other moments (+)= 0
+ *   kr_scattering_zonal_adjoint  its transpose (phi from phi_out)
+ * Gradients into per-PR-chunk arrays in sigt_zonal layout (fwd and adj
+ * share a context and a shape; fwd holds psi' = H^-1 rhs and phi, adj holds
+ * rhsbar and phi_outbar of the same iteration):
+ *   kr_grad_sigt        g(g,z) (+)= -sum over the chunk's subdomains, d of adj.rhs * fwd.psi
+ *   kr_grad_sigs_zonal  g(g,z) (+)=  adj.phi_out(0,g,z) * fwd.phi(0,g,z)
+ * Pointer arrays are host arrays of num_pr device pointers. */
+int kr_ltimes_adjoint(kr_problem *prob);
+int kr_lplustimes_adjoint(kr_problem *prob);
+int kr_scattering_adjoint(kr_problem *prob, int32_t accumulate);
+int kr_scattering_zonal(kr_problem *prob, const double *const *sigs_zonal, int32_t accumulate);
+int kr_scattering_zonal_adjoint(kr_problem *prob, const double *const *sigs_zonal, int32_t accumulate);
+int kr_sweep_solver_adjoint(kr_problem *prob);
+int kr_grad_sigt(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
+int kr_grad_sigs_zonal(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
+
 /* ---- per-entry-point timing (hipEvents on the context stream) ---------- */
 /* Names: LTimes, LPlusTimes, Scattering, Source, SweepSubdomain,
  * SweepSolver, Population, kConst.  kr_problem_timer synchronizes. */
@@ -195,6 +228,9 @@ kr_problem *kr_host_kernel_problem(kr_host_problem *hp);
  * *num_elem; dst may be NULL to query).  Integer fields convert to double. */
 int kr_host_field_get(kr_host_problem *hp, const char *name, int32_t chunk, double *dst, int64_t *num_elem);
 int kr_host_field_set(kr_host_problem *hp, const char *name, int32_t chunk, const double *src);
+/* Device address and element count of a double field's chunk (device-
+ * resident problems), for callers that move data on the device. */
+int kr_host_field_device(kr_host_problem *hp, const char *name, int32_t chunk, void **ptr, int64_t *num_elem);
 /* SteadyStateSolver (src/Kripke/SteadyStateSolver.cpp:24-115).
  * mode 0: the reference call sequence (kConst fills, accumulating entry
  *         points, host-side SweepComm queue over kr_sweep_subdomain).

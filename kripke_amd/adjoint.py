@@ -1,0 +1,187 @@
+"""Reverse-mode gradient of K source iterations (no reference counterpart).
+
+The forward model is K steady-state iterations from psi = 0
+(src/Kripke/SteadyStateSolver.cpp:44-104) with the per-zone parameters
+made explicit:
+
+    phi_n      = L psi_n
+    phi_out_n  = S(s) phi_n + q            per-zone isotropic scattering s(g,z)
+    rhs_n      = L+ phi_out_n
+    psi_{n+1}  = H(sigt)^-1 rhs_n          the diamond-difference sweep
+    phi_K      = L psi_K                   the output
+
+with sigt(g,z) the sigma_t field (sigt_zonal) and s(g,z) the group-diagonal
+n = 0 scattering cross section, both one array per PR chunk in sigt_zonal
+layout.  Since psi_{n+1} depends on psi_n only through phi_n, a checkpoint is
+the (small) phi_n.
+
+Backward, for a cotangent phibar_K of phi_K:
+
+    psibar_K = L^T phibar_K
+    for n = K-1 .. 0:   (phi_n and psi_{n+1} recomputed from the nearest checkpoint)
+        rhsbar_n     = H^-T psibar_{n+1}                 adjoint sweep
+        d sigt      += -sum_{d} rhsbar_n * psi_{n+1}
+        phi_outbar_n = (L+)^T rhsbar_n
+        d s         += phi_outbar_n(0) * phi_n(0)
+        phibar_n     = S^T phi_outbar_n
+        psibar_n     = L^T phibar_n
+
+All of it runs in the library's HIP kernels through the C-ABI; this module only
+sequences the calls and owns the device buffers.  The adjoint variables live
+in a second problem with the same inputs (each adjoint in the field of its
+primal variable), so no field is copied in the backward chain.
+"""
+import ctypes
+
+import numpy as np
+
+from .capi import HostProblem, check, lib
+
+
+class DeviceBuffer:
+    """One kr_malloc'd device array of doubles."""
+
+    def __init__(self, ctx, n):
+        self.ctx, self.n = ctx, int(n)
+        self.ptr = ctypes.c_void_p()
+        check(lib().kr_malloc(ctx.h, max(self.n, 1) * 8, ctypes.byref(self.ptr)), "kr_malloc")
+
+    def free(self):
+        if self.ptr:
+            lib().kr_free(self.ctx.h, self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
+def _copy(ctx, dst, src, n):
+    check(lib().kr_kcopy(ctx.h, ctypes.c_void_p(dst), ctypes.c_void_p(src), int(n)), "kr_kcopy")
+
+
+def _fill(ctx, dst, n, v):
+    check(lib().kr_kconst(ctx.h, ctypes.c_void_p(dst), int(n), float(v)), "kr_kconst")
+
+
+class DifferentiableSolver:
+    """K source iterations as a function of (sigt, s) with phi checkpoints
+    every `checkpoint_every` iterations."""
+
+    def __init__(self, ctx, inp, iterations, checkpoint_every=2):
+        if iterations < 1 or checkpoint_every < 1:
+            raise ValueError("iterations and checkpoint_every must be >= 1")
+        self.ctx, self.K, self.ck = ctx, int(iterations), int(checkpoint_every)
+        self.fwd = HostProblem(inp, ctx)
+        self.adj = HostProblem(inp, ctx)
+        self.kf, self.ka = self.fwd.kernels(), self.adj.kernels()
+        self.npr, self.nsdom = self.fwd.num_pr, self.fwd.num_sdoms
+        self.sdoms = list(range(self.nsdom))
+        # per PR chunk: element counts of the sigma_t and phi chunks
+        self.nparam = [self.fwd.device_ptr("sigt_zonal", c)[1] for c in range(self.npr)]
+        self.nphi = [self.fwd.device_ptr("phi", c)[1] for c in range(self.npr)]
+        self._s = [DeviceBuffer(ctx, n) for n in self.nparam]        # s(g,z) actually used
+        self._gt = [DeviceBuffer(ctx, n) for n in self.nparam]       # gradient w.r.t. sigt
+        self._gs = [DeviceBuffer(ctx, n) for n in self.nparam]       # gradient w.r.t. s
+        nck = (self.K + self.ck - 1) // self.ck
+        self._ckpt = [[DeviceBuffer(ctx, n) for n in self.nphi] for _ in range(nck)]
+        self.forward_done = False
+
+    # ---- parameters ------------------------------------------------------
+    def set_params(self, sigt_ptrs, s_ptrs):
+        """Device arrays (one per PR chunk, sigt_zonal layout) of sigma_t and s:
+        copied into both problems' sigt_zonal and into the solver's s."""
+        for c in range(self.npr):
+            for hp in (self.fwd, self.adj):
+                _copy(self.ctx, hp.device_ptr("sigt_zonal", c)[0], sigt_ptrs[c], self.nparam[c])
+            _copy(self.ctx, self._s[c].ptr.value, s_ptrs[c], self.nparam[c])
+        self.forward_done = False
+
+    def param_ptrs(self, which):
+        """Device addresses of the problem-held parameters ('sigt', 's') or the
+        gradients ('grad_sigt', 'grad_s')."""
+        if which == "sigt":
+            return [self.fwd.device_ptr("sigt_zonal", c)[0] for c in range(self.npr)]
+        bufs = {"s": self._s, "grad_sigt": self._gt, "grad_s": self._gs}[which]
+        return [b.ptr.value for b in bufs]
+
+    # ---- forward ---------------------------------------------------------
+    def _tail(self):
+        """From fwd.phi = phi_n: phi_out, rhs and psi_{n+1} (phi left as phi_n)."""
+        self.kf.scattering_zonal(self.param_ptrs("s"))
+        self.kf.source(self.fwd.inp.source_strength)
+        self.kf.lplustimes(accumulate=False)
+        self.kf.sweep_solver(self.sdoms)
+
+    def _step(self):
+        """phi_n -> phi_{n+1} in fwd.phi."""
+        self._tail()
+        self.kf.ltimes(accumulate=False)
+
+    def _set_phi(self, bufs):
+        for c in range(self.npr):
+            _copy(self.ctx, self.fwd.device_ptr("phi", c)[0], bufs[c].ptr.value, self.nphi[c])
+
+    def forward(self):
+        """Runs the K iterations from psi = 0; phi_K is left in the forward
+        problem's phi (phi_ptrs())."""
+        for c in range(self.npr):
+            _fill(self.ctx, self.fwd.device_ptr("phi", c)[0], self.nphi[c], 0.0)  # phi_0 = L 0
+        for n in range(self.K):
+            if n % self.ck == 0:
+                for c in range(self.npr):
+                    _copy(self.ctx, self._ckpt[n // self.ck][c].ptr.value, self.fwd.device_ptr("phi", c)[0],
+                          self.nphi[c])
+            self._step()
+        self.forward_done = True
+
+    def phi_ptrs(self, problem="fwd"):
+        hp = self.fwd if problem == "fwd" else self.adj
+        return [hp.device_ptr("phi", c)[0] for c in range(self.npr)]
+
+    # ---- backward --------------------------------------------------------
+    def backward(self, phibar_ptrs):
+        """Gradient of <phibar, phi_K> w.r.t. sigt and s, left in
+        param_ptrs('grad_sigt') and param_ptrs('grad_s')."""
+        if not self.forward_done:
+            raise RuntimeError("backward needs the forward pass of the same parameters")
+        for c in range(self.npr):
+            _copy(self.ctx, self.adj.device_ptr("phi", c)[0], phibar_ptrs[c], self.nphi[c])
+            _fill(self.ctx, self._gt[c].ptr.value, self.nparam[c], 0.0)
+            _fill(self.ctx, self._gs[c].ptr.value, self.nparam[c], 0.0)
+        self.ka.ltimes_adjoint()                                  # psibar_K
+        gt, gs = self.param_ptrs("grad_sigt"), self.param_ptrs("grad_s")
+        for n in range(self.K - 1, -1, -1):
+            c0 = (n // self.ck) * self.ck
+            self._set_phi(self._ckpt[n // self.ck])               # phi_c0
+            for _ in range(c0, n):
+                self._step()                                      # phi_n
+            self._tail()                                          # psi_{n+1}; phi still phi_n
+            self.ka.sweep_solver_adjoint()                        # rhsbar_n
+            self.kf.grad_sigt(self.ka, gt, accumulate=True)
+            self.ka.lplustimes_adjoint()                          # phi_outbar_n
+            self.kf.grad_sigs_zonal(self.ka, gs, accumulate=True)
+            self.ka.scattering_zonal(self.param_ptrs("s"), adjoint=True)  # phibar_n
+            self.ka.ltimes_adjoint()                              # psibar_n
+
+    def synchronize(self):
+        self.ctx.synchronize()
+
+    def close(self):
+        for b in self._s + self._gt + self._gs:
+            b.free()
+        for ck in self._ckpt:
+            for b in ck:
+                b.free()
+        self.fwd.close()
+        self.adj.close()
+
+
+def chunk_get(ctx, ptr, n):
+    """Device array -> numpy (synchronizes)."""
+    out = np.empty(int(n), dtype=np.float64)
+    check(lib().kr_memcpy_dtoh(ctx.h, out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), int(n) * 8),
+          "kr_memcpy_dtoh")
+    return out
+
+
+def chunk_set(ctx, ptr, values):
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    check(lib().kr_memcpy_htod(ctx.h, ctypes.c_void_p(ptr), v.ctypes.data_as(ctypes.c_void_p), v.size * 8),
+          "kr_memcpy_htod")

@@ -100,6 +100,14 @@ SIGNATURES = [
     ("kr_sweep_solver", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     ("kr_sweep_solver_population", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, DP]),
     ("kr_population", ctypes.c_int, [P, DP]),
+    ("kr_ltimes_adjoint", ctypes.c_int, [P]),
+    ("kr_lplustimes_adjoint", ctypes.c_int, [P]),
+    ("kr_scattering_adjoint", ctypes.c_int, [P, ctypes.c_int32]),
+    ("kr_scattering_zonal", ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_int32]),
+    ("kr_scattering_zonal_adjoint", ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_int32]),
+    ("kr_sweep_solver_adjoint", ctypes.c_int, [P]),
+    ("kr_grad_sigt", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
+    ("kr_grad_sigs_zonal", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_problem_set_timing", ctypes.c_int, [P, ctypes.c_int32]),
     ("kr_problem_timer", ctypes.c_int, [P, ctypes.c_char_p, DP, ctypes.POINTER(ctypes.c_int64)]),
     ("kr_problem_timer_reset", ctypes.c_int, [P]),
@@ -110,6 +118,8 @@ SIGNATURES = [
     ("kr_host_kernel_problem", P, [P]),
     ("kr_host_field_get", ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int32, DP, ctypes.POINTER(ctypes.c_int64)]),
     ("kr_host_field_set", ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int32, DP]),
+    ("kr_host_field_device", ctypes.c_int, [P, ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(P),
+                                             ctypes.POINTER(ctypes.c_int64)]),
     ("kr_host_init_psi", ctypes.c_int, [P]),
     ("kr_host_iterate", ctypes.c_int, [P, ctypes.c_int32, DP]),
     ("kr_host_solve", ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, DP]),
@@ -214,6 +224,32 @@ class KernelProblem:
         check(lib().kr_population(self.h, ctypes.byref(v)), "kr_population")
         return v.value
 
+    # ---- transposed operators and parameter gradients (include/kripke_amd.h)
+    def ltimes_adjoint(self):
+        check(lib().kr_ltimes_adjoint(self.h), "kr_ltimes_adjoint")
+
+    def lplustimes_adjoint(self):
+        check(lib().kr_lplustimes_adjoint(self.h), "kr_lplustimes_adjoint")
+
+    def scattering_adjoint(self, accumulate=False):
+        check(lib().kr_scattering_adjoint(self.h, 1 if accumulate else 0), "kr_scattering_adjoint")
+
+    def scattering_zonal(self, ptrs, accumulate=False, adjoint=False):
+        arr = (P * len(ptrs))(*ptrs)
+        fn = lib().kr_scattering_zonal_adjoint if adjoint else lib().kr_scattering_zonal
+        check(fn(self.h, arr, 1 if accumulate else 0), "kr_scattering_zonal")
+
+    def sweep_solver_adjoint(self):
+        check(lib().kr_sweep_solver_adjoint(self.h), "kr_sweep_solver_adjoint")
+
+    def grad_sigt(self, adj, ptrs, accumulate=False):
+        arr = (P * len(ptrs))(*ptrs)
+        check(lib().kr_grad_sigt(self.h, adj.h, arr, 1 if accumulate else 0), "kr_grad_sigt")
+
+    def grad_sigs_zonal(self, adj, ptrs, accumulate=False):
+        arr = (P * len(ptrs))(*ptrs)
+        check(lib().kr_grad_sigs_zonal(self.h, adj.h, arr, 1 if accumulate else 0), "kr_grad_sigs_zonal")
+
     def set_timing(self, on=True):
         check(lib().kr_problem_set_timing(self.h, 1 if on else 0), "kr_problem_set_timing")
 
@@ -261,6 +297,13 @@ class HostProblem:
     def set(self, name, chunk, values):
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().kr_host_field_set(self.h, name.encode(), chunk, v.ctypes.data_as(DP)), "field_set " + name)
+
+    def device_ptr(self, name, chunk=0):
+        """(device address, element count) of a double field's chunk."""
+        ptr, n = P(), ctypes.c_int64()
+        check(lib().kr_host_field_device(self.h, name.encode(), chunk, ctypes.byref(ptr), ctypes.byref(n)),
+              "field_device " + name)
+        return ptr.value, n.value
 
     def init_psi(self):
         check(lib().kr_host_init_psi(self.h), "init_psi")

@@ -137,6 +137,21 @@ int kr_host_field_set(kr_host_problem *hp, const char *name, int32_t chunk, cons
   });
 }
 
+int kr_host_field_device(kr_host_problem *hp, const char *name, int32_t chunk, void **ptr, int64_t *n) {
+  if (!hp || !name || !ptr) return KR_ERR_INVALID;
+  return guarded([&] {
+    auto &f = hp->ds.getVariable<Field<double>>(name);
+    if (chunk < 0 || (size_t)chunk >= f.numChunks() || !f.onDevice()) {
+      kr::set_error("chunk out of range or field not on the device");
+      return KR_ERR_INVALID;
+    }
+    const SdomId s = f.chunkSdom((size_t)chunk);
+    *ptr = f.device(s);
+    if (n) *n = (int64_t)f.size(s);
+    return KR_OK;
+  });
+}
+
 int kr_host_init_psi(kr_host_problem *hp) {
   if (!hp || !hp->ctx) {
     kr::set_error("kr_host_init_psi: problem has no device");

@@ -50,6 +50,10 @@ struct kr_problem {
   int32_t units_tmp_cap = 0;
   double *d_pop_units = nullptr;               // fused population: one partial per sweep unit
   int64_t pop_units_cap = 0;
+  SweepPlanes *d_planes_adj = nullptr;         // adjoint sweep: faces from the forward-downwind side
+  static constexpr int PTR_SLOTS = 4;          // zonal sigma_s, sigma_t gradient, sigma_s gradient, spare
+  const double **d_ptrs = nullptr;             // PTR_SLOTS per-PR-chunk pointer arrays passed to kernels
+  std::vector<const double *> ptrs_last[PTR_SLOTS];  // their current contents
   // population
   void *d_pop_blocks = nullptr;
   int64_t pop_nblocks = 0;
@@ -179,6 +183,9 @@ int check_unit_spans(const kr_problem *p, const std::vector<SweepUnit> &units, c
       if (off(last, S.pa[x], S.pg[x]) - off(first, S.pa[x], S.pg[x]) + (nf[x] - 1) * S.pf[x] + 1 > lim)
         return fail(KR_ERR_UNSUPPORTED, "sweep: a work unit's plane slice exceeds 2 GiB");
     if ((int64_t)S.ng * S.nz > lim) return fail(KR_ERR_UNSUPPORTED, "sweep: sigma_t chunk exceeds 2 GiB");
+    const int64_t strides[] = {S.fa, S.fg, S.fz, S.sg, S.sz, S.pa[0], S.pa[1], S.pa[2], S.pg[0], S.pg[1], S.pg[2]};
+    for (int64_t st : strides)
+      if (st >= (1LL << 31)) return fail(KR_ERR_UNSUPPORTED, "sweep: a field stride exceeds 2^31 elements");
   }
   return KR_OK;
 }
@@ -492,7 +499,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   }
 
   // --- sweep planes tables --------------------------------------------------
-  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom);
+  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom), padj(p->nsdom);
   for (int s = 0; s < p->nsdom; ++s) {
     const DevSdom &H = p->h_sd[s];
     for (int dim = 0; dim < 3; ++dim) {
@@ -500,6 +507,10 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
       pref[s].out[dim] = H.plane[dim];
       ppipe[s].in[dim] = H.up[dim] >= 0 ? H.plane[dim] : nullptr;
       ppipe[s].out[dim] = H.down[dim] >= 0 ? p->h_sd[H.down[dim]].plane[dim] : H.plane[dim];
+      // the adjoint sweep runs against the flow: its faces enter from the
+      // forward-downwind neighbour and leave toward the forward-upwind one
+      padj[s].in[dim] = H.down[dim] >= 0 ? H.plane[dim] : nullptr;
+      padj[s].out[dim] = H.up[dim] >= 0 ? p->h_sd[H.up[dim]].plane[dim] : H.plane[dim];
     }
   }
 
@@ -543,7 +554,8 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   p->pop_nblocks = population_block_table(p->h_sd.data(), p->nsdom, popb);
   if ((rc = upload(ctx, &p->d_sd, hsd)) || (rc = upload(ctx, &p->d_pr, p->h_pr)) || (rc = upload(ctx, &p->d_r, hr)) ||
       (rc = upload(ctx, &p->d_pr_sdoms, pr_sdoms)) || (rc = upload(ctx, &p->d_planes_ref, pref)) ||
-      (rc = upload(ctx, &p->d_planes_pipe, ppipe)) || (rc = upload(ctx, &p->d_units_ref, uref)) ||
+      (rc = upload(ctx, &p->d_planes_pipe, ppipe)) || (rc = upload(ctx, &p->d_planes_adj, padj)) ||
+      (rc = upload(ctx, &p->d_units_ref, uref)) ||
       (rc = upload(ctx, &p->d_units_pipe, upipe))) {
     kr_problem_destroy(p);
     return rc;
@@ -571,7 +583,8 @@ int kr_problem_destroy(kr_problem *p) {
   }
   void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
                   p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
-                  p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units};
+                  p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units,
+                  p->d_planes_adj,  (void *)p->d_ptrs};
   for (void *b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -718,6 +731,124 @@ int kr_sweep_solver_population(kr_problem *p, const int32_t *list, int32_t num, 
     KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
     return KR_OK;
   }
+}
+
+// ---- adjoint (transposed) operators and parameter gradients ---------------
+
+// Device copy of a per-PR-chunk pointer array (re-uploaded only when it
+// changes; the stream is drained first so no running kernel reads it).
+static int put_ptrs(kr_problem *p, int slot, const void *const *ptrs, const double **&dev) {
+  std::vector<const double *> v(p->npr);
+  for (int c = 0; c < p->npr; ++c) {
+    if (!ptrs[c]) return fail(KR_ERR_INVALID, "null chunk pointer");
+    v[c] = (const double *)ptrs[c];
+  }
+  const int n = std::max(1, p->npr);
+  if (!p->d_ptrs) {
+    KR_HIP_CHECK(hipMalloc((void **)&p->d_ptrs, sizeof(double *) * n * kr_problem::PTR_SLOTS));
+  }
+  dev = p->d_ptrs + (size_t)slot * n;
+  if (v == p->ptrs_last[slot]) return KR_OK;
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  KR_HIP_CHECK(hipMemcpy(dev, v.data(), sizeof(double *) * p->npr, hipMemcpyHostToDevice));
+  p->ptrs_last[slot] = v;
+  return KR_OK;
+}
+
+static void sigt_layout(const kr_problem *p, int64_t *s_g, int64_t *s_z) {
+  const DevSdom &S = p->h_sd[0];
+  sigt_strides(p->nesting, S.ng, S.nz, s_g, s_z);
+}
+
+int kr_ltimes_adjoint(kr_problem *p) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_ltimes_adjoint: null problem");
+  CallTimer t(p, "LTimesAdjoint");
+  return launch_ltimes_adjoint(launch_ctx(p));
+}
+
+int kr_lplustimes_adjoint(kr_problem *p) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_lplustimes_adjoint: null problem");
+  CallTimer t(p, "LPlusTimesAdjoint");
+  return launch_lplustimes_adjoint(launch_ctx(p));
+}
+
+int kr_scattering_adjoint(kr_problem *p, int32_t accumulate) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_scattering_adjoint: null problem");
+  CallTimer t(p, "ScatteringAdjoint");
+  return launch_scattering(launch_ctx(p), p->pr_gset.data(), p->pr_r.data(), accumulate ? 1 : 0, true);
+}
+
+int kr_scattering_zonal(kr_problem *p, const double *const *sigs_zonal, int32_t accumulate) {
+  if (!p || !sigs_zonal) return fail(KR_ERR_INVALID, "kr_scattering_zonal: null argument");
+  const double **dev = nullptr;
+  int rc = put_ptrs(p, 0, (const void *const *)sigs_zonal, dev);
+  if (rc) return rc;
+  CallTimer t(p, "Scattering");
+  int64_t sg, sz;
+  sigt_layout(p, &sg, &sz);
+  return launch_scattering_zonal(launch_ctx(p), dev, sg, sz, accumulate ? 1 : 0, false);
+}
+
+int kr_scattering_zonal_adjoint(kr_problem *p, const double *const *sigs_zonal, int32_t accumulate) {
+  if (!p || !sigs_zonal) return fail(KR_ERR_INVALID, "kr_scattering_zonal_adjoint: null argument");
+  const double **dev = nullptr;
+  int rc = put_ptrs(p, 0, (const void *const *)sigs_zonal, dev);
+  if (rc) return rc;
+  CallTimer t(p, "ScatteringAdjoint");
+  int64_t sg, sz;
+  sigt_layout(p, &sg, &sz);
+  return launch_scattering_zonal(launch_ctx(p), dev, sg, sz, accumulate ? 1 : 0, true);
+}
+
+int kr_sweep_solver_adjoint(kr_problem *p) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_sweep_solver_adjoint: null problem");
+  CallTimer t(p, "SweepSolverAdjoint");
+  int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, 0, p->nsdom);
+  if (rc) return rc;
+  for (size_t n = p->pipe_level_begin.size(); n-- > 0;) {  // the forward stages in reverse order
+    CallTimer tk(p, "k_sweep_adjoint");
+    rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_adj, p->d_units_pipe + p->pipe_level_begin[n],
+                      p->pipe_level_count[n], p->shape_pipe, nullptr, true);
+    if (rc) return rc;
+  }
+  return KR_OK;
+}
+
+static int same_shape(const kr_problem *a, const kr_problem *b) {
+  if (!a || !b) return fail(KR_ERR_INVALID, "null problem");
+  if (a->ctx != b->ctx) return fail(KR_ERR_INVALID, "forward and adjoint problems must share a context");
+  if (a->nesting != b->nesting || a->nsdom != b->nsdom || a->npr != b->npr || a->M != b->M)
+    return fail(KR_ERR_INVALID, "forward and adjoint problems differ in shape");
+  for (int s = 0; s < a->nsdom; ++s) {
+    const DevSdom &x = a->h_sd[s], &y = b->h_sd[s];
+    if (x.nd != y.nd || x.ng != y.ng || x.ni != y.ni || x.nj != y.nj || x.nk != y.nk || x.pr != y.pr)
+      return fail(KR_ERR_INVALID, "forward and adjoint problems differ in shape");
+  }
+  return KR_OK;
+}
+
+int kr_grad_sigt(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate) {
+  int rc = same_shape(fwd, adj);
+  if (rc) return rc;
+  if (!grad) return fail(KR_ERR_INVALID, "kr_grad_sigt: null gradient array");
+  const double **dev = nullptr;
+  if ((rc = put_ptrs(fwd, 1, (const void *const *)grad, dev))) return rc;
+  CallTimer t(fwd, "GradSigt");
+  int64_t sg, sz;
+  sigt_layout(fwd, &sg, &sz);
+  return launch_grad_sigt(launch_ctx(fwd), adj->d_sd, (double *const *)dev, sg, sz, accumulate ? 1 : 0);
+}
+
+int kr_grad_sigs_zonal(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate) {
+  int rc = same_shape(fwd, adj);
+  if (rc) return rc;
+  if (!grad) return fail(KR_ERR_INVALID, "kr_grad_sigs_zonal: null gradient array");
+  const double **dev = nullptr;
+  if ((rc = put_ptrs(fwd, 2, (const void *const *)grad, dev))) return rc;
+  CallTimer t(fwd, "GradSigs");
+  int64_t sg, sz;
+  sigt_layout(fwd, &sg, &sz);
+  return launch_grad_sigs_zonal(launch_ctx(fwd), adj->d_pr, (double *const *)dev, sg, sz, accumulate ? 1 : 0);
 }
 
 int kr_population(kr_problem *p, double *out) {

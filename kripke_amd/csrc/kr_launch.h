@@ -29,7 +29,19 @@ constexpr int SIGS_FLAG_BLOCKS = 64;
 
 int launch_ltimes(const LaunchCtx &L, int accumulate);
 int launch_lplustimes(const LaunchCtx &L, int accumulate);
-int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate);
+int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate,
+                      bool adjoint = false);
+// per-zone isotropic group-diagonal scattering from s(g,z) arrays (device array of one pointer per PR chunk)
+int launch_scattering_zonal(const LaunchCtx &L, const double *const *sz_dev, int64_t s_g, int64_t s_z, int accumulate,
+                            bool adjoint);
+// adjoints of the moment maps: psi = L^T phi; phi_out = (L+)^T rhs
+int launch_ltimes_adjoint(const LaunchCtx &L);
+int launch_lplustimes_adjoint(const LaunchCtx &L);
+// parameter gradients (F: forward problem; adj_*: the adjoint problem's device tables)
+int launch_grad_sigt(const LaunchCtx &F, const DevSdom *adj_sd, double *const *grad_dev, int64_t s_g, int64_t s_z,
+                     int accumulate);
+int launch_grad_sigs_zonal(const LaunchCtx &F, const DevPR *adj_pr, double *const *grad_dev, int64_t s_g, int64_t s_z,
+                           int accumulate);
 int launch_source(const LaunchCtx &L, double strength);
 int launch_population(const LaunchCtx &L, const void *blocks, int64_t num_blocks, double *partial, double *out);
 int64_t population_block_table(const DevSdom *host_sd, int nsdom, std::vector<char> &bytes);
@@ -38,12 +50,14 @@ int launch_copy(hipStream_t st, double *d, const double *s, int64_t n);
 
 // Sweep: one launch sweeps every unit; units of one launch share `shape`.
 // pop (row-scan kernel only): one population partial per unit, or nullptr.
+// adjoint: the transposed sweep (reads psi, writes rhs, axes reversed; planes
+// and units of the reversed dependency order).
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                 int32_t num_units, const SweepShape &shape, double *pop = nullptr);
+                 int32_t num_units, const SweepShape &shape, double *pop = nullptr, bool adjoint = false);
 // Row-scan sweep of the zone-fastest nestings (kr_sweep_rows.hip).
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                       int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
-                      double *pop);
+                      double *pop, bool adjoint);
 // Sum of n doubles in a fixed order (one workgroup): out[0] = sum(partial[0..n)).
 int launch_sum(hipStream_t st, const double *partial, int64_t n, double *out);
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).

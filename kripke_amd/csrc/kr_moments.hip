@@ -13,6 +13,14 @@
 
 namespace kr {
 
+// Blocks for n items at per_block per block, capped (grid-stride kernels).
+static inline int grid_for(int64_t n, int per_block = 256, int max_blocks = 8192) {
+  int64_t b = (n + per_block - 1) / per_block;
+  if (b > max_blocks) b = max_blocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
 // Map a flat (group, zone) index to (g, z) so that consecutive threads are
 // consecutive in memory for the field's stride pattern.
 __device__ inline void gz_of(int64_t e, int32_t ng, int32_t nz, bool zfast, int32_t &g, int32_t &z) {
@@ -46,7 +54,9 @@ __device__ __forceinline__ void ltimes_batch(double (&acc)[M], const gmem<double
   }
 }
 
-template <int M, int DB, bool ACC>
+// ADJ: the same contraction as LPlusTimes' transpose: phi_out(nm) (+)= sum_d
+// ell_plus(d,nm) rhs(d) (the adjoint of LPlusTimes maps rhs to phi_out).
+template <int M, int DB, bool ACC, bool ADJ>
 __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
                                                 const int32_t *__restrict__ pr_sdoms) {
   const DevPR &P = prs[blockIdx.y];
@@ -60,14 +70,14 @@ __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, 
   int32_t g, z;
   gz_of(e, P.ng, P.nz, zfast, g, z);
   const int64_t po = g * P.mg + z * P.mz;
-  gmem<double> *phi = gp(P.phi);
+  gmem<double> *phi = gp(ADJ ? P.phi_out : P.phi);
   double acc[M];
 #pragma unroll
   for (int nm = 0; nm < M; ++nm) acc[nm] = ACC ? phi[po + nm * P.ma] : 0.0;
   for (int k = 0; k < P.sdom_count; ++k) {
     const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    const gmem<double> *psi = gp(S.psi) + g * S0.fg + z * S0.fz;
-    const cmem<double> *ell = cp(S.ell);
+    const gmem<double> *psi = gp(ADJ ? S.rhs : S.psi) + g * S0.fg + z * S0.fz;
+    const cmem<double> *ell = cp(ADJ ? S.ell_plus : S.ell);
     for (int d0 = 0; d0 < nd; d0 += DB) ltimes_batch<M, DB>(acc, psi + d0 * fa, fa, ell + d0 * M);  // DB divides nd
   }
 #pragma unroll
@@ -96,7 +106,8 @@ __device__ __forceinline__ void lplus_batch(const double (&pho)[M], gmem<double>
   for (int b = 0; b < DB; ++b) __builtin_nontemporal_store(acc[b], rhs + b * fa);
 }
 
-template <int M, int DB, bool ACC>
+// ADJ: LTimes' transpose: psi(d) (+)= sum_nm ell(d,nm) phi(nm).
+template <int M, int DB, bool ACC, bool ADJ>
 __global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
                                                     const int32_t *__restrict__ pr_sdoms) {
   const DevPR &P = prs[blockIdx.y];
@@ -110,14 +121,14 @@ __global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ 
   int32_t g, z;
   gz_of(e, P.ng, P.nz, zfast, g, z);
   const int64_t po = g * P.mg + z * P.mz;
-  const gmem<double> *pout = gp(P.phi_out);
+  const gmem<double> *pout = gp(ADJ ? P.phi : P.phi_out);
   double pho[M];
 #pragma unroll
   for (int nm = 0; nm < M; ++nm) pho[nm] = pout[po + nm * P.ma];
   for (int k = 0; k < P.sdom_count; ++k) {
     const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    gmem<double> *rhs = gp(S.rhs) + g * S0.fg + z * S0.fz;
-    const cmem<double> *elp = cp(S.ell_plus);
+    gmem<double> *rhs = gp(ADJ ? S.psi : S.rhs) + g * S0.fg + z * S0.fz;
+    const cmem<double> *elp = cp(ADJ ? S.ell : S.ell_plus);
     for (int d0 = 0; d0 < nd; d0 += DB) lplus_batch<M, DB, ACC>(pho, rhs + d0 * fa, fa, elp + d0 * M);  // DB divides nd
   }
 }
@@ -174,7 +185,7 @@ __device__ __forceinline__ void tile_to_slab(gmem<double> *dst, const double *ti
   }
 }
 
-template <int M, bool ACC>
+template <int M, bool ACC, bool ADJ>
 __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restrict__ sd, const DevPR *__restrict__ prs,
                                                         const int32_t *__restrict__ pr_sdoms, int ts) {
   extern __shared__ double tile[];
@@ -185,7 +196,7 @@ __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restric
   const int cn = (int)min<int64_t>(DF_CB, C - c0);
   const int tid = threadIdx.x;
   const int nd = sd[pr_sdoms[P.sdom_begin]].nd;
-  gmem<double> *phi = gp(P.phi) + c0 * M;
+  gmem<double> *phi = gp(ADJ ? P.phi_out : P.phi) + c0 * M;
   double acc[M];
   if (ACC) {
     slab_to_tile<false>(tile, ts, phi, cn * M, M);
@@ -199,9 +210,9 @@ __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restric
   }
   for (int k = 0; k < P.sdom_count; ++k) {
     const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    slab_to_tile<true>(tile, ts, gp(S.psi) + c0 * nd, cn * nd, nd);
+    slab_to_tile<true>(tile, ts, gp(ADJ ? S.rhs : S.psi) + c0 * nd, cn * nd, nd);
     __syncthreads();
-    const cmem<double> *ell = cp(S.ell);
+    const cmem<double> *ell = cp(ADJ ? S.ell_plus : S.ell);
     if (tid < cn) {
       for (int d = 0; d < nd; ++d) {
         const double p = tile[tid * ts + d];
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restric
   tile_to_slab(phi, tile, ts, cn * M, M);
 }
 
-template <int M, bool ACC>
+template <int M, bool ACC, bool ADJ>
 __global__ __launch_bounds__(DF_CB) void k_lplustimes_dfast(const DevSdom *__restrict__ sd,
                                                             const DevPR *__restrict__ prs,
                                                             const int32_t *__restrict__ pr_sdoms, int ts) {
@@ -229,7 +240,7 @@ __global__ __launch_bounds__(DF_CB) void k_lplustimes_dfast(const DevSdom *__res
   const int cn = (int)min<int64_t>(DF_CB, C - c0);
   const int tid = threadIdx.x;
   const int nd = sd[pr_sdoms[P.sdom_begin]].nd;
-  slab_to_tile<false>(tile, ts, gp(P.phi_out) + c0 * M, cn * M, M);
+  slab_to_tile<false>(tile, ts, gp(ADJ ? P.phi : P.phi_out) + c0 * M, cn * M, M);
   __syncthreads();
   double pho[M];
 #pragma unroll
@@ -237,12 +248,12 @@ __global__ __launch_bounds__(DF_CB) void k_lplustimes_dfast(const DevSdom *__res
   __syncthreads();
   for (int k = 0; k < P.sdom_count; ++k) {
     const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    gmem<double> *rhs = gp(S.rhs) + c0 * nd;
+    gmem<double> *rhs = gp(ADJ ? S.psi : S.rhs) + c0 * nd;
     if (ACC) {
       slab_to_tile<false>(tile, ts, rhs, cn * nd, nd);
       __syncthreads();
     }
-    const cmem<double> *elp = cp(S.ell_plus);
+    const cmem<double> *elp = cp(ADJ ? S.ell : S.ell_plus);
     if (tid < cn) {
       for (int d = 0; d < nd; ++d) {
         double acc = ACC ? tile[tid * ts + d] : 0.0;
@@ -308,6 +319,8 @@ __device__ inline bool sigs_coupled(const int32_t *__restrict__ flags) {
 // Structure-aware path for isotropic, group-diagonal sigs (the generated
 // data, src/Kripke/Generate/Data.cpp:87-101): every skipped product is an
 // exact zero, so the result equals the dense loop bit for bit.
+// ADJ: the transpose, phi (+)= S^T phi_out; a diagonal S is its own transpose.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
                                                       const double *__restrict__ sigs, SigsStrides ss, int M,
                                                       int accumulate, const int32_t *__restrict__ flags) {
@@ -322,18 +335,24 @@ __global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ 
     const int64_t gg = P.glo + g;
     const double s = sigs_zone(sigs, ss, R, z, 0, gg, gg);
     const int64_t po = g * P.mg + z * P.mz;
-    const double base0 = accumulate ? P.phi_out[po] : 0.0;
-    const double t = s * P.phi[po];
-    P.phi_out[po] = base0 + t;
+    double *dst = ADJ ? P.phi : P.phi_out;
+    const double *src = ADJ ? P.phi_out : P.phi;
+    const double base0 = accumulate ? dst[po] : 0.0;
+    const double t = s * src[po];
+    dst[po] = base0 + t;
     for (int nm = 1; nm < M; ++nm) {
-      const double b = accumulate ? P.phi_out[po + nm * P.ma] : 0.0;
-      P.phi_out[po + nm * P.ma] = b + 0.0;
+      const double b = accumulate ? dst[po + nm * P.ma] : 0.0;
+      dst[po + nm * P.ma] = b + 0.0;
     }
   }
 }
 
 // General sigs (group coupling, anisotropic moments): one thread per
 // (moment, group, zone) of the destination chunk, source groups in order.
+// ADJ: the transpose for the same (src, dst) chunk pair: phi_src(nm,gp,z)
+// (+)= sum over the destination chunk's groups g of sigs(n, g, gp) *
+// phi_out_dst(nm,g,z); the thread owns (nm, gp, z) of the source chunk.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void k_scatter_dense(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
                                                        const double *__restrict__ sigs, SigsStrides ss,
                                                        const int64_t *__restrict__ m2l, int M, int src, int dst,
@@ -342,23 +361,120 @@ __global__ __launch_bounds__(256) void k_scatter_dense(const DevPR *__restrict__
   const DevPR &Ps = prs[src];
   const DevPR &Pd = prs[dst];
   const DevR &R = rs[Pd.r];
-  const int64_t n = (int64_t)M * Pd.ng * Pd.nz;
-  const bool zfast = Pd.mz <= Pd.mg;
+  const DevPR &Po = ADJ ? Ps : Pd;  // chunk owning the outputs
+  const DevPR &Pi = ADJ ? Pd : Ps;  // chunk read
+  const int64_t n = (int64_t)M * Po.ng * Po.nz;
+  const bool zfast = Po.mz <= Po.mg;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t per = (int64_t)Pd.ng * Pd.nz;
+    const int64_t per = (int64_t)Po.ng * Po.nz;
     const int nm = (int)(e / per);
     int32_t g, z;
-    gz_of(e % per, Pd.ng, Pd.nz, zfast, g, z);
+    gz_of(e % per, Po.ng, Po.nz, zfast, g, z);
     const int64_t nl = m2l[nm];
-    const int64_t od = nm * Pd.ma + g * Pd.mg + z * Pd.mz;
-    double acc = accumulate ? Pd.phi_out[od] : 0.0;
-    for (int32_t gp = 0; gp < Ps.ng; ++gp) {
-      const double s = sigs_zone(sigs, ss, R, z, nl, Pd.glo + g, Ps.glo + gp);
-      const double t = s * Ps.phi[nm * Ps.ma + gp * Ps.mg + z * Ps.mz];
+    const int64_t od = nm * Po.ma + g * Po.mg + z * Po.mz;
+    double *out = ADJ ? Po.phi : Po.phi_out;
+    const double *in = ADJ ? Pi.phi_out : Pi.phi;
+    double acc = accumulate ? out[od] : 0.0;
+    for (int32_t h = 0; h < Pi.ng; ++h) {
+      const double s = ADJ ? sigs_zone(sigs, ss, R, z, nl, Pd.glo + h, Ps.glo + g)
+                           : sigs_zone(sigs, ss, R, z, nl, Pd.glo + g, Ps.glo + h);
+      const double t = s * in[nm * Pi.ma + h * Pi.mg + z * Pi.mz];
       acc = acc + t;
     }
-    Pd.phi_out[od] = acc;
+    out[od] = acc;
   }
+}
+
+// Scattering with a per-zone, isotropic, group-diagonal cross section s(g,z)
+// given directly (one array per PR chunk in sigt_zonal layout) instead of
+// through materials: phi_out(0,g,z) (+)= s(g,z)*phi(0,g,z), higher moments
+// (+)= 0.  The form the differentiable solver uses for per-zone sigma_s; for
+// s equal to the mixed-element average of the generated sigs it matches
+// k_scatter_diag.  ADJ: the transpose (phi from phi_out), the same map.
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_scatter_zonal(const DevPR *__restrict__ prs, const double *const *__restrict__ sz,
+                                                       int64_t s_g, int64_t s_z, int M, int accumulate) {
+  const DevPR &P = prs[blockIdx.y];
+  const double *sc = sz[blockIdx.y];
+  const int64_t n = (int64_t)P.ng * P.nz;
+  const bool zfast = P.mz <= P.mg;
+  double *dst = ADJ ? P.phi : P.phi_out;
+  const double *src = ADJ ? P.phi_out : P.phi;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, P.ng, P.nz, zfast, g, z);
+    const int64_t po = g * P.mg + z * P.mz;
+    const double t = sc[g * s_g + z * s_z] * src[po];
+    dst[po] = (accumulate ? dst[po] : 0.0) + t;
+    for (int nm = 1; nm < M; ++nm) dst[po + nm * P.ma] = (accumulate ? dst[po + nm * P.ma] : 0.0) + 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Parameter gradients of the differentiable source iteration (no reference
+// counterpart).  With psi' = H(sigt)^-1 rhs, d psi'/d sigt(g,z) = -H^-1 diag
+// (psi'), so dJ/d sigt(g,z) = -sum_{subdomains, d} rhsbar(d,g,z) psi'(d,g,z),
+// where rhsbar = H^-T psibar is what the adjoint sweep leaves in the adjoint
+// problem's rhs.  For the zonal scattering phi_out(0) = s phi(0):
+// dJ/ds(g,z) = phi_outbar(0,g,z) phi(0,g,z).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_grad_sigt(const DevSdom *__restrict__ fsd, const DevSdom *__restrict__ asd,
+                                                   const DevPR *__restrict__ prs, const int32_t *__restrict__ pr_sdoms,
+                                                   double *const *__restrict__ grad, int64_t s_g, int64_t s_z,
+                                                   int accumulate) {
+  const DevPR &P = prs[blockIdx.y];
+  const int64_t n = (int64_t)P.ng * P.nz;
+  const DevSdom &S0 = fsd[pr_sdoms[P.sdom_begin]];
+  const bool zfast = S0.fz <= S0.fg;
+  double *gr = grad[blockIdx.y];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, P.ng, P.nz, zfast, g, z);
+    const int64_t o = g * S0.fg + z * S0.fz;
+    double acc = 0.0;
+    for (int k = 0; k < P.sdom_count; ++k) {
+      const int sdom = pr_sdoms[P.sdom_begin + k];
+      const gmem<double> *pf = gp(fsd[sdom].psi) + o;
+      const gmem<double> *ra = gp(asd[sdom].rhs) + o;
+      for (int d = 0; d < S0.nd; ++d) acc = __fma_rn(ra[d * S0.fa], pf[d * S0.fa], acc);
+    }
+    const int64_t go = g * s_g + z * s_z;
+    gr[go] = (accumulate ? gr[go] : 0.0) - acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grad_sigs_zonal(const DevPR *__restrict__ fpr, const DevPR *__restrict__ apr,
+                                                         double *const *__restrict__ grad, int64_t s_g, int64_t s_z,
+                                                         int accumulate) {
+  const DevPR &F = fpr[blockIdx.y];
+  const DevPR &A = apr[blockIdx.y];
+  const int64_t n = (int64_t)F.ng * F.nz;
+  const bool zfast = F.mz <= F.mg;
+  double *gr = grad[blockIdx.y];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, F.ng, F.nz, zfast, g, z);
+    const int64_t po = g * F.mg + z * F.mz;
+    const int64_t go = g * s_g + z * s_z;
+    gr[go] = (accumulate ? gr[go] : 0.0) + A.phi_out[po] * F.phi[po];
+  }
+}
+
+int launch_grad_sigt(const LaunchCtx &F, const DevSdom *adj_sd, double *const *grad_dev, int64_t s_g, int64_t s_z,
+                     int accumulate) {
+  dim3 grid(grid_for(F.max_gz), F.num_pr);
+  hipLaunchKernelGGL(k_grad_sigt, grid, dim3(256), 0, F.stream, F.sd, adj_sd, F.pr, F.pr_sdoms, grad_dev, s_g, s_z,
+                     accumulate);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_grad_sigs_zonal(const LaunchCtx &F, const DevPR *adj_pr, double *const *grad_dev, int64_t s_g, int64_t s_z,
+                           int accumulate) {
+  dim3 grid(grid_for(F.max_gz), F.num_pr);
+  hipLaunchKernelGGL(k_grad_sigs_zonal, grid, dim3(256), 0, F.stream, F.pr, adj_pr, grad_dev, s_g, s_z, accumulate);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -454,12 +570,6 @@ __global__ __launch_bounds__(256) void k_copy(double *__restrict__ d, const doub
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static inline int grid_for(int64_t n, int per_block = 256, int max_blocks = 8192) {
-  int64_t b = (n + per_block - 1) / per_block;
-  if (b > max_blocks) b = max_blocks;
-  if (b < 1) b = 1;
-  return (int)b;
-}
 
 #define KR_MOMENT_CASES(X) X(1) X(4) X(9) X(16) X(25) X(36) X(49) X(64)
 
@@ -476,75 +586,75 @@ static int batch_of(int nd, int cap) {
 static int dfast_stride(int nd, int M) { return std::max(nd, M) | 1; }
 static bool dfast_fits(int nd, int M) { return (size_t)DF_CB * dfast_stride(nd, M) * sizeof(double) <= 65536; }
 
-template <int M, bool ACC>
+template <int M, bool ACC, bool ADJ>
 static void ltimes_launch(const LaunchCtx &L, dim3 grid, int db) {
   if (L.dfast && dfast_fits(L.nd, M)) {
     const int ts = dfast_stride(L.nd, M);
     dim3 g((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
-    hipLaunchKernelGGL((k_ltimes_dfast<M, ACC>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double), L.stream,
+    hipLaunchKernelGGL((k_ltimes_dfast<M, ACC, ADJ>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double), L.stream,
                        L.sd, L.pr, L.pr_sdoms, ts);
     return;
   }
   switch (db) {
-    case 16: hipLaunchKernelGGL((k_ltimes<M, 16, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
-    case 8: hipLaunchKernelGGL((k_ltimes<M, 8, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
-    case 4: hipLaunchKernelGGL((k_ltimes<M, 4, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
-    case 2: hipLaunchKernelGGL((k_ltimes<M, 2, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
-    default: hipLaunchKernelGGL((k_ltimes<M, 1, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 16: hipLaunchKernelGGL((k_ltimes<M, 16, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 8: hipLaunchKernelGGL((k_ltimes<M, 8, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 4: hipLaunchKernelGGL((k_ltimes<M, 4, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 2: hipLaunchKernelGGL((k_ltimes<M, 2, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    default: hipLaunchKernelGGL((k_ltimes<M, 1, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
   }
 }
 
-template <int M, bool ACC>
+template <int M, bool ACC, bool ADJ>
 static void lplus_launch(const LaunchCtx &L, dim3 grid, int db) {
   if (L.dfast && dfast_fits(L.nd, M)) {
     const int ts = dfast_stride(L.nd, M);
     dim3 g((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
-    hipLaunchKernelGGL((k_lplustimes_dfast<M, ACC>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double), L.stream,
-                       L.sd, L.pr, L.pr_sdoms, ts);
+    hipLaunchKernelGGL((k_lplustimes_dfast<M, ACC, ADJ>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double),
+                       L.stream, L.sd, L.pr, L.pr_sdoms, ts);
     return;
   }
   switch (db) {
-    case 4: hipLaunchKernelGGL((k_lplustimes<M, 4, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
-    case 2: hipLaunchKernelGGL((k_lplustimes<M, 2, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
-    default: hipLaunchKernelGGL((k_lplustimes<M, 1, ACC>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 4: hipLaunchKernelGGL((k_lplustimes<M, 4, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    case 2: hipLaunchKernelGGL((k_lplustimes<M, 2, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
+    default: hipLaunchKernelGGL((k_lplustimes<M, 1, ACC, ADJ>), grid, dim3(256), 0, L.stream, L.sd, L.pr, L.pr_sdoms); break;
   }
 }
 
-int launch_ltimes(const LaunchCtx &L, int accumulate) {
+// mode: 0 overwrite, 1 accumulate, 2 adjoint (overwrite)
+static int ltimes_shape(const LaunchCtx &L, int mode, const char *what, bool lplus) {
   dim3 grid(grid_for(L.max_gz, 256, 1 << 30), L.num_pr);
-  const int db = batch_of(L.nd, 16);
+  const int db = batch_of(L.nd, lplus ? 4 : 16);
   switch (L.num_moments) {
-#define X(m)                                          \
-  case m:                                             \
-    if (accumulate) ltimes_launch<m, true>(L, grid, db); \
-    else ltimes_launch<m, false>(L, grid, db);          \
+#define X(m)                                                     \
+  case m:                                                        \
+    if (lplus) {                                                 \
+      if (mode == 1) lplus_launch<m, true, false>(L, grid, db);  \
+      else if (mode == 0) lplus_launch<m, false, false>(L, grid, db); \
+      else lplus_launch<m, false, true>(L, grid, db);            \
+    } else {                                                     \
+      if (mode == 1) ltimes_launch<m, true, false>(L, grid, db); \
+      else if (mode == 0) ltimes_launch<m, false, false>(L, grid, db); \
+      else ltimes_launch<m, false, true>(L, grid, db);           \
+    }                                                            \
     break;
     KR_MOMENT_CASES(X)
 #undef X
-    default: set_error("LTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
+    default: set_error(std::string(what) + ": unsupported number of moments"); return KR_ERR_UNSUPPORTED;
   }
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }
 
+int launch_ltimes(const LaunchCtx &L, int accumulate) { return ltimes_shape(L, accumulate ? 1 : 0, "LTimes", false); }
 int launch_lplustimes(const LaunchCtx &L, int accumulate) {
-  dim3 grid(grid_for(L.max_gz, 256, 1 << 30), L.num_pr);
-  const int db = batch_of(L.nd, 4);
-  switch (L.num_moments) {
-#define X(m)                                         \
-  case m:                                            \
-    if (accumulate) lplus_launch<m, true>(L, grid, db); \
-    else lplus_launch<m, false>(L, grid, db);          \
-    break;
-    KR_MOMENT_CASES(X)
-#undef X
-    default: set_error("LPlusTimes: unsupported number of moments"); return KR_ERR_UNSUPPORTED;
-  }
-  KR_HIP_CHECK(hipGetLastError());
-  return KR_OK;
+  return ltimes_shape(L, accumulate ? 1 : 0, "LPlusTimes", true);
 }
+// psi = L^T phi (the LPlusTimes contraction with ell)
+int launch_ltimes_adjoint(const LaunchCtx &L) { return ltimes_shape(L, 2, "LTimes adjoint", true); }
+// phi_out = (L+)^T rhs (the LTimes contraction with ell_plus)
+int launch_lplustimes_adjoint(const LaunchCtx &L) { return ltimes_shape(L, 2, "LPlusTimes adjoint", false); }
 
-int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate) {
+int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate, bool adjoint) {
   SigsStrides ss{L.sigs_str[0], L.sigs_str[1], L.sigs_str[2], L.sigs_str[3]};
   // 1) classify data/sigs as it is now; 2) the diagonal path and the dense
   // path are both enqueued and each returns at once unless it applies.
@@ -552,22 +662,46 @@ int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t 
                      L.sigs_dims[1], L.sigs_dims[2], L.sigs_flags);
   KR_HIP_CHECK(hipGetLastError());
   dim3 grid(grid_for(L.max_gz), L.num_pr);
-  hipLaunchKernelGGL(k_scatter_diag, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments, accumulate,
-                     (const int32_t *)L.sigs_flags);
+  if (adjoint)
+    hipLaunchKernelGGL(k_scatter_diag<true>, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments,
+                       accumulate, (const int32_t *)L.sigs_flags);
+  else
+    hipLaunchKernelGGL(k_scatter_diag<false>, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments,
+                       accumulate, (const int32_t *)L.sigs_flags);
   KR_HIP_CHECK(hipGetLastError());
   // Dense: for each source chunk (work-list order) and each destination
-  // chunk on the same zone set (Scattering.cpp:137-145).
+  // chunk on the same zone set (Scattering.cpp:137-145).  The transpose
+  // writes the source chunks: the first pair that writes a chunk overwrites
+  // (unless accumulating), later pairs add.
   std::vector<int> first(L.num_pr, 1);
   for (int s = 0; s < L.num_pr; ++s)
     for (int d = 0; d < L.num_pr; ++d) {
       if (pr_r[s] != pr_r[d]) continue;
-      const int acc = first[d] ? accumulate : 1;
-      first[d] = 0;
-      hipLaunchKernelGGL(k_scatter_dense, dim3(grid_for(L.max_gz * L.num_moments)), dim3(256), 0, L.stream, L.pr,
-                         L.r, L.sigs, ss, L.m2l, L.num_moments, s, d, acc, (const int32_t *)L.sigs_flags);
+      const int w = adjoint ? s : d;
+      const int acc = first[w] ? accumulate : 1;
+      first[w] = 0;
+      if (adjoint)
+        hipLaunchKernelGGL(k_scatter_dense<true>, dim3(grid_for(L.max_gz * L.num_moments)), dim3(256), 0, L.stream,
+                           L.pr, L.r, L.sigs, ss, L.m2l, L.num_moments, s, d, acc, (const int32_t *)L.sigs_flags);
+      else
+        hipLaunchKernelGGL(k_scatter_dense<false>, dim3(grid_for(L.max_gz * L.num_moments)), dim3(256), 0, L.stream,
+                           L.pr, L.r, L.sigs, ss, L.m2l, L.num_moments, s, d, acc, (const int32_t *)L.sigs_flags);
       KR_HIP_CHECK(hipGetLastError());
     }
   (void)pr_gset;
+  return KR_OK;
+}
+
+int launch_scattering_zonal(const LaunchCtx &L, const double *const *sz_dev, int64_t s_g, int64_t s_z, int accumulate,
+                            bool adjoint) {
+  dim3 grid(grid_for(L.max_gz), L.num_pr);
+  if (adjoint)
+    hipLaunchKernelGGL(k_scatter_zonal<true>, grid, dim3(256), 0, L.stream, L.pr, sz_dev, s_g, s_z, L.num_moments,
+                       accumulate);
+  else
+    hipLaunchKernelGGL(k_scatter_zonal<false>, grid, dim3(256), 0, L.stream, L.pr, sz_dev, s_g, s_z, L.num_moments,
+                       accumulate);
+  KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }
 

@@ -66,9 +66,12 @@ struct ColArgs {
   double *out[3];         // destination of faces leaving the subdomain
   int32_t has_in[3];      // 0: vacuum face (reads as zero)
   double *pop;
-  int64_t fa, fg, fz, sg, sz;
-  int64_t pa[3], pg[3], pf[3];
-  int64_t base_f, base_p[3];  // element offset of the unit's first pair in psi/rhs and in the planes
+  // strides (elements; each below 2^31) and the unit's first pair (d0, g0):
+  // offsets are taken relative to that pair, where the resources are based
+  int32_t fa, fg, fz, sg, sz;
+  int32_t pa[3], pg[3], pf[3];
+  int32_t d0, g0;
+  int64_t base_f, base_p[3];  // element offset of (d0, g0) in psi/rhs and in the planes
   int32_t nd, ng, ni, nj, nk, jd, kd;
   int32_t pg_begin, R, inner_is_g;
 };
@@ -110,13 +113,15 @@ __device__ __forceinline__ void col_patch(ColCursor &c, const ColArgs &P, const 
   c.j = P.jd > 0 ? jl : P.nj - 1 - jl;
   c.k = P.kd > 0 ? kl : P.nk - 1 - kl;
   c.i = K.istep > 0 ? 0 : P.ni - 1;
-  const int64_t z = ((int64_t)c.k * P.nj + c.j) * P.ni + c.i;
-  c.psi8 = (uint32_t)(8 * (c.d * P.fa + c.g * P.fg - P.base_f + z * P.fz));
-  c.sig8 = (uint32_t)(8 * (c.g * P.sg + z * P.sz));
-  c.x8 = (uint32_t)(8 * (c.d * P.pa[0] + c.g * P.pg[0] - P.base_p[0] + ((int64_t)c.k * P.nj + c.j) * P.pf[0]));
-  c.y8 = (uint32_t)(8 * (c.d * P.pa[1] + c.g * P.pg[1] - P.base_p[1] + ((int64_t)c.k * P.ni + c.i) * P.pf[1]));
-  c.z8 = (uint32_t)(8 * (c.d * P.pa[2] + c.g * P.pg[2] - P.base_p[2] + ((int64_t)c.j * P.ni + c.i) * P.pf[2]));
-  c.vol8 = (uint32_t)(8 * z);
+  // relative offsets stay below 2 GiB (host-checked unit spans): 32-bit math
+  const uint32_t z = ((uint32_t)c.k * P.nj + c.j) * P.ni + c.i;
+  const int dd = c.d - P.d0, dg = c.g - P.g0;
+  c.psi8 = 8u * (uint32_t)(dd * P.fa + dg * P.fg + (int)z * P.fz);
+  c.sig8 = 8u * (uint32_t)(c.g * P.sg + (int)z * P.sz);
+  c.x8 = 8u * (uint32_t)(dd * P.pa[0] + dg * P.pg[0] + (c.k * P.nj + c.j) * P.pf[0]);
+  c.y8 = 8u * (uint32_t)(dd * P.pa[1] + dg * P.pg[1] + (c.k * P.ni + c.i) * P.pf[1]);
+  c.z8 = 8u * (uint32_t)(dd * P.pa[2] + dg * P.pg[2] + (c.j * P.ni + c.i) * P.pf[2]);
+  c.vol8 = 8u * z;
 }
 
 __device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const ColConst &K, int pos) {
@@ -151,7 +156,7 @@ __device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const C
   col_patch(c, P, K);
 }
 
-template <bool POP>
+template <bool POP, bool ADJ>
 __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, double *lds) {
   constexpr int RING = COL_RING;
   const int nt = blockDim.x;
@@ -200,6 +205,8 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   const Rsrc Rout2 = rsrc(P.out[2] + P.base_p[2], span(items * nf[2] - P.base_p[2]));
 
   double rq[RING], rs[RING], rx[RING], ry[RING], rz[RING];
+#pragma unroll
+  for (int r = 0; r < RING; ++r) rx[r] = ry[r] = rz[r] = 0.0;
   ColCursor lc, cc;
   col_init(lc, P, K, -(K.jj + K.kk));
   cc = lc;
@@ -210,9 +217,9 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     // incoming faces: x at a pencil start; y for the first j-row of a patch
     // (the subdomain's incoming face on the first patch, else the previous
     // patch's outgoing face in the own chunk); z likewise
-    rx[r] = ld<0>(Rin0, lc.ii == 0 ? lc.x8 : NOWHERE);
-    ry[r] = ld<0>(Rown1, (K.jj == 0 && (lc.pj > 0 || P.has_in[1])) ? lc.y8 : NOWHERE);
-    rz[r] = ld<0>(Rown2, (K.kk == 0 && (lc.pk > 0 || P.has_in[2])) ? lc.z8 : NOWHERE);
+    if (lc.ii == 0) rx[r] = ld<0>(Rin0, lc.x8);
+    if (K.jj == 0) ry[r] = ld<0>(Rown1, (lc.pj > 0 || P.has_in[1]) ? lc.y8 : NOWHERE);
+    if (K.kk == 0) rz[r] = ld<0>(Rown2, (lc.pk > 0 || P.has_in[2]) ? lc.z8 : NOWHERE);
     col_step(lc, P, K);
   };
 
@@ -237,20 +244,28 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       if (cc.ii == 0) fx = rx[r];
       const double fy = K.jj == 0 ? ry[r] : fyb[prev * nt + tid - K.Q];
       const double fzv = K.kk == 0 ? rz[r] : fzb[prev * nt + tid - K.Q * K.PJ];
-      const double num = __fma_rn(fzv, ck, __fma_rn(fy, bj, __fma_rn(fx, a, rq[r])));
+      // adjoint: incoming faces weigh 2, outgoing faces a, b, c (transposed zone relations)
+      const double wx = ADJ ? 2.0 : a, wy = ADJ ? 2.0 : bj, wz = ADJ ? 2.0 : ck;
+      const double num = __fma_rn(fzv, wz, __fma_rn(fy, wy, __fma_rn(fx, wx, rq[r])));
       const double den = ((a + bj) + ck) + rs[r];
       const double p = num / den;
       const double p2 = 2.0 * p;
-      fx = p2 - fx;
-      const double fyo = p2 - fy, fzo = p2 - fzv;
+      fx = (ADJ ? a * p : p2) - fx;
+      const double fyo = (ADJ ? bj * p : p2) - fy, fzo = (ADJ ? ck * p : p2) - fzv;
       st<NT>(Rpsi, act ? cc.psi8 : NOWHERE, p);
-      st<0>(Rout0, (act && cc.ii == ni - 1) ? cc.x8 : NOWHERE, fx);
-      const bool yedge = act && K.jj == K.PJ - 1, zedge = act && K.kk == K.PK - 1;
-      const bool ylast = cc.pj == K.NPJ - 1, zlast = cc.pk == K.NPK - 1;
-      st<0>(Rout1, (yedge && ylast) ? cc.y8 : NOWHERE, fyo);
-      st<0>(Rown1, (yedge && !ylast) ? cc.y8 : NOWHERE, fyo);
-      st<0>(Rout2, (zedge && zlast) ? cc.z8 : NOWHERE, fzo);
-      st<0>(Rown2, (zedge && !zlast) ? cc.z8 : NOWHERE, fzo);
+      if (act && cc.ii == ni - 1) st<0>(Rout0, cc.x8, fx);
+      if (act && K.jj == K.PJ - 1) {
+        if (cc.pj == K.NPJ - 1)
+          st<0>(Rout1, cc.y8, fyo);
+        else
+          st<0>(Rown1, cc.y8, fyo);
+      }
+      if (act && K.kk == K.PK - 1) {
+        if (cc.pk == K.NPK - 1)
+          st<0>(Rout2, cc.z8, fzo);
+        else
+          st<0>(Rown2, cc.z8, fzo);
+      }
       fyb[cur * nt + tid] = fyo;
       fzb[cur * nt + tid] = fzo;
       if (POP) {
@@ -278,7 +293,8 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
 
 }  // namespace
 
-template <bool POP>
+// ADJ: the transposed sweep (reads psi, writes rhs, every axis reversed).
+template <bool POP, bool ADJ>
 __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes,
                                                 const SweepUnit *__restrict__ units, SweepShape sh,
                                                 double *__restrict__ pop) {
@@ -287,44 +303,44 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   const DevSdom &S = sd[U.sdom];
   const SweepPlanes &PL = planes[U.sdom];
   ColArgs P;
-  P.rhs = S.rhs;
+  P.rhs = ADJ ? S.psi : S.rhs;
   P.sig = S.sig;
   P.coef = S.coef[0];
   P.w = S.w;
   P.vol = S.volume;
-  P.psi = S.psi;
+  P.psi = ADJ ? S.rhs : S.psi;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
     P.own[x] = S.plane[x];
     P.out[x] = PL.out[x];
     P.has_in[x] = PL.in[x] != nullptr;
-    P.pa[x] = S.pa[x];
-    P.pg[x] = S.pg[x];
-    P.pf[x] = S.pf[x];
+    P.pa[x] = (int32_t)S.pa[x];
+    P.pg[x] = (int32_t)S.pg[x];
+    P.pf[x] = (int32_t)S.pf[x];
   }
   P.pop = POP ? pop + blockIdx.x : nullptr;
-  P.fa = S.fa;
-  P.fg = S.fg;
-  P.fz = S.fz;
-  P.sg = S.sg;
-  P.sz = S.sz;
+  P.fa = (int32_t)S.fa;
+  P.fg = (int32_t)S.fg;
+  P.fz = (int32_t)S.fz;
+  P.sg = (int32_t)S.sg;
+  P.sz = (int32_t)S.sz;
   P.nd = S.nd;
   P.ng = S.ng;
   P.ni = S.ni;
   P.nj = S.nj;
   P.nk = S.nk;
-  P.jd = S.jd;
-  P.kd = S.kd;
+  P.jd = ADJ ? -S.jd : S.jd;
+  P.kd = ADJ ? -S.kd : S.kd;
   P.pg_begin = U.pg_begin;
   P.R = U.pg_count;
   P.inner_is_g = sh.inner_is_g;
   {
     const int pidx0 = U.pg_begin * sh.Q;  // the unit's first pair: smallest offsets in every field
-    const int d0 = sh.inner_is_g ? pidx0 / S.ng : pidx0 % S.nd;
-    const int g0 = sh.inner_is_g ? pidx0 % S.ng : pidx0 / S.nd;
-    P.base_f = d0 * S.fa + g0 * S.fg;
+    P.d0 = sh.inner_is_g ? pidx0 / S.ng : pidx0 % S.nd;
+    P.g0 = sh.inner_is_g ? pidx0 % S.ng : pidx0 / S.nd;
+    P.base_f = P.d0 * S.fa + P.g0 * S.fg;
 #pragma unroll
-    for (int x = 0; x < 3; ++x) P.base_p[x] = d0 * S.pa[x] + g0 * S.pg[x];
+    for (int x = 0; x < 3; ++x) P.base_p[x] = P.d0 * S.pa[x] + P.g0 * S.pg[x];
   }
   ColConst K;
   K.Q = sh.Q;
@@ -336,13 +352,13 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   K.q = tid % sh.Q;
   K.jj = (tid / sh.Q) % sh.PJ;
   K.kk = tid / (sh.Q * sh.PJ);
-  K.istep = S.id > 0 ? 1 : -1;
+  K.istep = ((S.id > 0) != ADJ) ? 1 : -1;
   K.dpsi8 = (int32_t)(8 * K.istep * S.fz);
   K.dsig8 = (int32_t)(8 * K.istep * S.sz);
   K.dy8 = (int32_t)(8 * K.istep * S.pf[1]);
   K.dz8 = (int32_t)(8 * K.istep * S.pf[2]);
   K.dvol8 = 8 * K.istep;
-  cols_body<POP>(P, K, lds);
+  cols_body<POP, ADJ>(P, K, lds);
 }
 
 // 2*cos(d)/delta per (direction, zone index), the exact expression of
@@ -462,16 +478,23 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
 }
 
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                 int32_t num_units, const SweepShape &shape, double *pop) {
+                 int32_t num_units, const SweepShape &shape, double *pop, bool adjoint) {
   if (num_units <= 0) return KR_OK;
+  if (adjoint && pop) {
+    set_error("sweep: no population in the adjoint sweep");
+    return KR_ERR_INVALID;
+  }
   if (shape.kind == 1)
-    return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk, pop);
+    return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk, pop,
+                             adjoint);
   const int nt = shape.Q * shape.PJ * shape.PK;
   const size_t lds = (4 * (size_t)nt + (size_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1)) * sizeof(double);
-  if (pop)
-    hipLaunchKernelGGL(k_sweep<true>, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
+  if (adjoint)
+    hipLaunchKernelGGL((k_sweep<false, true>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
+  else if (pop)
+    hipLaunchKernelGGL((k_sweep<true, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
   else
-    hipLaunchKernelGGL(k_sweep<false>, dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
+    hipLaunchKernelGGL((k_sweep<false, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }

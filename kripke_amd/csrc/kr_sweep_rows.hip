@@ -226,7 +226,7 @@ __device__ __forceinline__ constexpr int memo(int m) {
   return SX > 0 ? m : V - 1 - m;  // memory slot of sweep position m within a lane's block
 }
 
-template <int W, int V, int SX, bool POP>
+template <int W, int V, int SX, bool POP, bool ADJ>
 __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   // steps of prefetch: 3 (2 with the population's extra registers)
   constexpr int RING = POP ? 2 : 3;
@@ -370,10 +370,13 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
         const int e = memo<SX, V>(m);
         const double den = (a[m] + bc) + rs[r][e];
         inv[m] = recip(den);
-        q[m] = __fma_rn(fz[m], ck, __fma_rn(fy[m], bj, rq[r][e]));
+        // adjoint: incoming faces weigh 2 and outgoing faces a, b, c (the
+        // transposed zone relations), so only q, B and the face updates change
+        q[m] = ADJ ? __fma_rn(fz[m], 2.0, __fma_rn(fy[m], 2.0, rq[r][e]))
+                   : __fma_rn(fz[m], ck, __fma_rn(fy[m], bj, rq[r][e]));
         const double i2 = inv[m] + inv[m];
         const double A = __fma_rn(a[m], i2, -1.0);
-        const double B = q[m] * i2;
+        const double B = ADJ ? (a[m] * q[m]) * inv[m] : q[m] * i2;
         if (m == 0) {
           Ac = A;
           Bc = B;
@@ -393,11 +396,18 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
 #pragma unroll
       for (int m = 0; m < V; ++m) {
         const int e = memo<SX, V>(m);
-        p[e] = __fma_rn(a[m], f, q[m]) * inv[m];
-        const double p2 = p[e] + p[e];
-        f = p2 - f;
-        fyo[e] = p2 - fy[m];
-        fzo[e] = p2 - fz[m];
+        if (ADJ) {
+          p[e] = __fma_rn(2.0, f, q[m]) * inv[m];
+          f = a[m] * p[e] - f;
+          fyo[e] = bj * p[e] - fy[m];
+          fzo[e] = ck * p[e] - fz[m];
+        } else {
+          p[e] = __fma_rn(a[m], f, q[m]) * inv[m];
+          const double p2 = p[e] + p[e];
+          f = p2 - f;
+          fyo[e] = p2 - fy[m];
+          fzo[e] = p2 - fz[m];
+        }
         fy[m] = fyo[e];
       }
 
@@ -440,7 +450,9 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
 
 }  // namespace
 
-template <int W, int V, bool POP>
+// ADJ: the transposed sweep, psi-shaped rhsbar = H^-T psibar: it reads the
+// psi field, writes the rhs field and runs every axis in reverse.
+template <int W, int V, bool POP, bool ADJ>
 __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__ sd,
                                                      const SweepPlanes *__restrict__ planes,
                                                      const SweepUnit *__restrict__ units, SweepShape sh,
@@ -450,10 +462,10 @@ __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__
   const DevSdom &S = sd[U.sdom];
   const SweepPlanes &PL = planes[U.sdom];
   RowArgs P;
-  P.rhs = S.rhs;
+  P.rhs = ADJ ? S.psi : S.rhs;
   P.sig = S.sig;
   P.coef = S.coef[0];
-  P.psi = S.psi;
+  P.psi = ADJ ? S.rhs : S.psi;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
     P.in[x] = PL.in[x];
@@ -464,23 +476,23 @@ __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__
   P.ni = S.ni;
   P.nj = S.nj;
   P.nk = S.nk;
-  P.jd = S.jd;
-  P.kd = S.kd;
+  P.jd = ADJ ? -S.jd : S.jd;
+  P.kd = ADJ ? -S.kd : S.kd;
   P.pg_begin = U.pg_begin;
   P.R = U.pg_count;
   P.inner_is_g = sh.inner_is_g;
   P.w = S.w;
   P.vol = S.volume;
   P.pop = POP ? pop + blockIdx.x : nullptr;
-  if (S.id > 0)
-    rows_body<W, V, 1, POP>(P, lds);
+  if ((S.id > 0) != ADJ)
+    rows_body<W, V, 1, POP, ADJ>(P, lds);
   else
-    rows_body<W, V, -1, POP>(P, lds);
+    rows_body<W, V, -1, POP, ADJ>(P, lds);
 }
 
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                       int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
-                      double *pop) {
+                      double *pop, bool adjoint) {
   if (num_units <= 0) return KR_OK;
   const int W = shape.RG, V = shape.V;
   const int nt = W * nk;
@@ -488,12 +500,15 @@ int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *plan
   switch (W * 8 + V) {
 #define X(w, v)                                                                                              \
   case w * 8 + v:                                                                                            \
-    if (pop)                                                                                                 \
-      hipLaunchKernelGGL((k_sweep_rows<w, v, true>), dim3(num_units), dim3(nt), lds, st, sd, planes, units,  \
-                         shape, pop);                                                                        \
+    if (adjoint)                                                                                             \
+      hipLaunchKernelGGL((k_sweep_rows<w, v, false, true>), dim3(num_units), dim3(nt), lds, st, sd, planes,  \
+                         units, shape, pop);                                                                 \
+    else if (pop)                                                                                            \
+      hipLaunchKernelGGL((k_sweep_rows<w, v, true, false>), dim3(num_units), dim3(nt), lds, st, sd, planes,  \
+                         units, shape, pop);                                                                 \
     else                                                                                                     \
-      hipLaunchKernelGGL((k_sweep_rows<w, v, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, \
-                         shape, pop);                                                                        \
+      hipLaunchKernelGGL((k_sweep_rows<w, v, false, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, \
+                         units, shape, pop);                                                                 \
     break;
     X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(64, 1)
     X(4, 2) X(8, 2) X(16, 2) X(32, 2) X(64, 2)

@@ -1,0 +1,252 @@
+"""Transposed operators and the reverse-mode gradient (GPU, through the C-ABI).
+
+Pins, each independent of the code under test:
+* transpose identities <y, A x> = <A^T y, x> on random FP64 vectors for
+  LTimes, LPlusTimes, scattering (structure-aware, dense and zonal) and the
+  pipelined sweep, in every nesting;
+* central finite differences of the forward solve (the same library, whose
+  forward is pinned to the oracle elsewhere) against the adjoint gradient;
+* the gradient does not depend on the checkpoint interval (bit for bit);
+* torch.autograd.gradcheck of the torch wrapper in FP64.
+"""
+import numpy as np
+import pytest
+
+from helpers import kwargs_for, product_for
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = ["DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD"]
+TOL_DOT = 1e-13
+
+
+def _rand_fields(hp, name, count, rng, offset=0.0):
+    vals = []
+    for c in range(count):
+        v = offset + rng.standard_normal(hp.get(name, c).size)
+        hp.set(name, c, v)
+        vals.append(v)
+    return vals
+
+
+def _get(hp, name, count):
+    return [hp.get(name, c) for c in range(count)]
+
+
+def _dot(a, b):
+    return float(sum(np.dot(x, y) for x, y in zip(a, b)))
+
+
+def _scale(a, b):
+    return float(sum(np.dot(np.abs(x), np.abs(y)) for x, y in zip(a, b)))
+
+
+def _check_dot(lhs_pairs, rhs_pairs):
+    lhs, rhs = _dot(*lhs_pairs), _dot(*rhs_pairs)
+    scale = max(_scale(*lhs_pairs), 1e-300)
+    assert abs(lhs - rhs) <= TOL_DOT * scale, "<y,Ax>=%.17g <A^T y,x>=%.17g (scale %.3g)" % (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_ltimes_transpose(ctx, layout):
+    hp = product_for(ctx, **kwargs_for(layout, zset=(2, 1, 1), gset=2, groups=8))
+    k = hp.kernels()
+    rng = np.random.default_rng(41)
+    x = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    k.ltimes(accumulate=False)
+    Lx = _get(hp, "phi", hp.num_pr)
+    y = _rand_fields(hp, "phi", hp.num_pr, rng)
+    k.ltimes_adjoint()
+    Lty = _get(hp, "psi", hp.num_sdoms)
+    _check_dot((y, Lx), (Lty, x))
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_lplustimes_transpose(ctx, layout):
+    hp = product_for(ctx, **kwargs_for(layout, zset=(1, 2, 1), gset=2, groups=8))
+    k = hp.kernels()
+    rng = np.random.default_rng(42)
+    x = _rand_fields(hp, "phi_out", hp.num_pr, rng)
+    k.lplustimes(accumulate=False)
+    Lx = _get(hp, "rhs", hp.num_sdoms)
+    y = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.lplustimes_adjoint()
+    Lty = _get(hp, "phi_out", hp.num_pr)
+    _check_dot((y, Lx), (Lty, x))
+
+
+@pytest.mark.parametrize("dense", [False, True])
+@pytest.mark.parametrize("layout", ["DGZ", "GZD", "ZDG"])
+def test_scattering_transpose(ctx, layout, dense):
+    hp = product_for(ctx, **kwargs_for(layout, zset=(1, 1, 2), gset=2, groups=4, legendre=1))
+    k = hp.kernels()
+    rng = np.random.default_rng(43)
+    if dense:  # group coupling and n = 1 terms: the dense kernels
+        hp.set("data/sigs", 0, rng.uniform(0.0, 0.1, hp.get("data/sigs").size))
+    x = _rand_fields(hp, "phi", hp.num_pr, rng)
+    k.scattering(accumulate=False)
+    Sx = _get(hp, "phi_out", hp.num_pr)
+    y = _rand_fields(hp, "phi_out", hp.num_pr, rng)
+    k.scattering_adjoint(accumulate=False)
+    Sty = _get(hp, "phi", hp.num_pr)
+    _check_dot((y, Sx), (Sty, x))
+
+
+def test_scattering_zonal_matches_materials(ctx):
+    """s(g,z) = the mixed-element average of the generated sigma_s: the zonal
+    kernel reproduces the structure-aware scattering bit for bit."""
+    from kripke_amd.adjoint import DeviceBuffer, chunk_set
+    hp = product_for(ctx, **kwargs_for("DGZ", zset=(2, 1, 1), groups=4))
+    k = hp.kernels()
+    rng = np.random.default_rng(44)
+    _rand_fields(hp, "phi", hp.num_pr, rng)
+    k.scattering(accumulate=False)
+    ref = _get(hp, "phi_out", hp.num_pr)
+    sigs = hp.get("data/sigs").reshape(3, -1, hp.inp.num_groups, hp.inp.num_groups)  # DGZ: [mat][n][g][gp]
+    bufs = []
+    for c in range(hp.num_pr):
+        r = c  # one group set: PR chunk c is zone set c
+        z2m = hp.get("zone_to_mixelem", r).astype(int)
+        z2n = hp.get("zone_to_num_mixelem", r).astype(int)
+        mat = hp.get("mixelem_to_material", r).astype(int)
+        frac = hp.get("mixelem_to_fraction", r)
+        nz, ng = z2m.size, hp.inp.num_groups
+        s = np.zeros((ng, nz))
+        for z in range(nz):
+            acc = np.zeros(ng)
+            for q in range(z2n[z]):
+                m = z2m[z] + q
+                acc = acc + np.diag(sigs[mat[m], 0]) * frac[m]
+            s[:, z] = acc
+        b = DeviceBuffer(ctx, s.size)
+        chunk_set(ctx, b.ptr.value, s.ravel())
+        bufs.append(b)
+    k.scattering_zonal([b.ptr.value for b in bufs])
+    for c in range(hp.num_pr):
+        assert np.array_equal(hp.get("phi_out", c), ref[c])
+    for b in bufs:
+        b.free()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("zones", [(8, 8, 8), (12, 6, 10)])
+def test_sweep_transpose(ctx, layout, zones):
+    hp = product_for(ctx, **kwargs_for(layout, zset=(2, 2, 2), groups=8, zones=zones))
+    k = hp.kernels()
+    rng = np.random.default_rng(45)
+    x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.sweep_solver(list(range(hp.num_sdoms)))
+    Hx = _get(hp, "psi", hp.num_sdoms)
+    y = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    k.sweep_solver_adjoint()
+    Hty = _get(hp, "rhs", hp.num_sdoms)
+    _check_dot((y, Hx), (Hty, x))
+
+
+# --------------------------------------------------------------------------
+# gradients
+# --------------------------------------------------------------------------
+def _solver(ctx, layout, iterations, ck):
+    import kripke_amd as ka
+    from kripke_amd.adjoint import DifferentiableSolver
+    inp = ka.input_vars(**kwargs_for(layout, zones=(6, 6, 6), zset=(2, 1, 1), groups=2, quad=(4, 2), legendre=1))
+    return DifferentiableSolver(ctx, inp, iterations, checkpoint_every=ck)
+
+
+def _params(sv, ctx):
+    from kripke_amd.adjoint import DeviceBuffer, chunk_get, chunk_set
+    sigt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("sigt"), sv.nparam)]
+    s = [0.6 * t for t in sigt]
+    bt = [DeviceBuffer(ctx, t.size) for t in sigt]
+    bs = [DeviceBuffer(ctx, t.size) for t in s]
+    for b, v in zip(bt + bs, sigt + s):
+        chunk_set(ctx, b.ptr.value, v)
+    return sigt, s, bt, bs
+
+
+def _loss(sv, ctx, w):
+    from kripke_amd.adjoint import chunk_get
+    return sum(float(np.dot(wc, chunk_get(ctx, p, n))) for wc, p, n in zip(w, sv.phi_ptrs(), sv.nphi))
+
+
+def _gradients(sv, ctx, bt, bs, w, wbufs):
+    from kripke_amd.adjoint import chunk_get
+    sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+    sv.forward()
+    sv.backward([b.ptr.value for b in wbufs])
+    gt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("grad_sigt"), sv.nparam)]
+    gs = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("grad_s"), sv.nparam)]
+    return gt, gs
+
+
+@pytest.mark.parametrize("layout", ["DGZ", "ZDG"])
+def test_gradient_vs_finite_differences(ctx, layout):
+    from kripke_amd.adjoint import DeviceBuffer, chunk_set
+    sv = _solver(ctx, layout, iterations=3, ck=2)
+    sigt, s, bt, bs = _params(sv, ctx)
+    rng = np.random.default_rng(46)
+    w = [rng.standard_normal(n) for n in sv.nphi]  # J = <w, phi_K>
+    wbufs = [DeviceBuffer(ctx, n) for n in sv.nphi]
+    for b, v in zip(wbufs, w):
+        chunk_set(ctx, b.ptr.value, v)
+    gt, gs = _gradients(sv, ctx, bt, bs, w, wbufs)
+    sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+    sv.forward()  # the backward pass leaves intermediate states in the forward problem
+    J0 = abs(_loss(sv, ctx, w))
+    for which, base, grad, bufs in (("sigt", sigt, gt, bt), ("s", s, gs, bs)):
+        for _ in range(6):
+            c = int(rng.integers(len(base)))
+            e = int(rng.integers(base[c].size))
+            h = 1e-6 * abs(base[c][e])
+            vals = []
+            for sgn in (1.0, -1.0):
+                v = base[c].copy()
+                v[e] += sgn * h
+                chunk_set(ctx, bufs[c].ptr.value, v)
+                sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+                sv.forward()
+                vals.append(_loss(sv, ctx, w))
+            chunk_set(ctx, bufs[c].ptr.value, base[c])
+            fd = (vals[0] - vals[1]) / (2 * h)
+            # central-difference truncation ~h^2, plus the rounding of J (~1e-13 |J|) over 2h
+            tol = 1e-6 * abs(fd) + 1e-13 * J0 / h
+            assert abs(fd - grad[c][e]) <= tol, \
+                "%s chunk %d entry %d: fd %.12e adjoint %.12e" % (which, c, e, fd, grad[c][e])
+    for b in bt + bs + wbufs:
+        b.free()
+    sv.close()
+
+
+def test_gradient_independent_of_checkpoint_interval(ctx):
+    from kripke_amd.adjoint import DeviceBuffer, chunk_set
+    ref = None
+    for ck in (1, 2, 4):
+        sv = _solver(ctx, "DGZ", iterations=4, ck=ck)
+        sigt, s, bt, bs = _params(sv, ctx)
+        w = [np.random.default_rng(48).standard_normal(n) for n in sv.nphi]
+        wbufs = [DeviceBuffer(ctx, n) for n in sv.nphi]
+        for b, v in zip(wbufs, w):
+            chunk_set(ctx, b.ptr.value, v)
+        gt, gs = _gradients(sv, ctx, bt, bs, w, wbufs)
+        if ref is None:
+            ref = (gt, gs)
+        else:
+            for a, b in zip(gt + gs, ref[0] + ref[1]):
+                assert np.array_equal(a, b)
+        for b in bt + bs + wbufs:
+            b.free()
+        sv.close()
+
+
+def test_torch_gradcheck():
+    """torch.autograd.gradcheck (FP64, fast mode) of the torch wrapper, in a
+    child process that imports torch before the library (one HIP runtime)."""
+    import os
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    case = os.path.join(os.path.dirname(os.path.abspath(__file__)), "torch_gradcheck_case.py")
+    r = subprocess.run([sys.executable, case], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=300)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-2000:]
