@@ -8,7 +8,9 @@ pipelined sweep and the population, over the workload
     Legendre 8 polar x 16 azimuthal, 8 direction sets), P4 scattering
     (25 moments), FP64, DGZ nesting, seeded 3-material box geometry.
 
-Prints one JSON line.  `value` is the reference figure of merit
+A second leg measures the metric's adjoint/forward cost on the
+differentiable solve (run_adjoint).  Prints one JSON line.  `value` is the
+reference figure of merit
 (src/kripke.cpp:495-505): unknowns (D*G*Z) per second per iteration, with all
 fields resident in HBM when the timed region starts.
 """
@@ -88,6 +90,76 @@ def run_gpu(args, layout):
     return elapsed, setup_s, timers, nb
 
 
+ADJ_WORKLOAD = dict(zones=(32, 32, 32), groups=32, quad=(8, 12), legendre=2, dset=8, gset=1, zset=(1, 1, 1),
+                    geometry=1, seed=20261202)
+ADJ_ITERATIONS, ADJ_CHECKPOINT = 10, 2
+
+
+def run_adjoint(layout):
+    """Adjoint/forward cost of the differentiable solve: 10 source iterations
+    of 32^3 zones x 32 groups x 96 directions (GL 8x12), P2, as a function of
+    per-zone sigma_t and sigma_s (one value per zone and group), loss =
+    sum over a seeded 4^3 detector box of (phi - phi_target)^2; phi
+    checkpoints every 2 iterations.  forward = the 10 iterations (with the
+    checkpoint copies); backward = the reverse sweep including the
+    recomputation from checkpoints."""
+    import numpy as np
+    import kripke_amd as ka
+    from kripke_amd.adjoint import DeviceBuffer, DifferentiableSolver, chunk_get, chunk_set
+    ctx = ka.Context(0)
+    sv = DifferentiableSolver(ctx, ka.input_vars(layout=layout, **ADJ_WORKLOAD), ADJ_ITERATIONS, ADJ_CHECKPOINT)
+    rng = np.random.default_rng(20261202)
+    sigt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("sigt"), sv.nparam)]
+    bt = [DeviceBuffer(ctx, n) for n in sv.nparam]
+    bs = [DeviceBuffer(ctx, n) for n in sv.nparam]
+    for b, v in zip(bt, sigt):
+        chunk_set(ctx, b.ptr.value, v)
+    for b, v in zip(bs, sigt):
+        chunk_set(ctx, b.ptr.value, 0.5 * v)
+    sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+    # detector box: 4^3 zones at a seeded corner; target phi = 0.9 x the phi of the unperturbed solve
+    nx = ADJ_WORKLOAD["zones"][0]
+    c0 = rng.integers(0, nx - 4, size=3)
+    sv.forward()
+    phi0 = [chunk_get(ctx, p, n) for p, n in zip(sv.phi_ptrs(), sv.nphi)]
+    target = [0.9 * v for v in phi0]
+    M, G = 9, ADJ_WORKLOAD["groups"]
+    mask = np.zeros((nx, nx, nx), dtype=bool)
+    mask[c0[0]:c0[0] + 4, c0[1]:c0[1] + 4, c0[2]:c0[2] + 4] = True
+    zmask = np.transpose(mask, (2, 1, 0)).ravel()  # zone index z = (k*ny + j)*nx + i
+    axes = {"DGZ": "agz", "DZG": "azg", "GDZ": "gaz", "GZD": "gza", "ZDG": "zag", "ZGD": "zga"}[layout]
+    shape = [{"a": M, "g": G, "z": zmask.size}[ch] for ch in axes]
+    zm = np.broadcast_to(zmask.reshape([zmask.size if ch == "z" else 1 for ch in axes]), shape).ravel()
+    wb = [DeviceBuffer(ctx, n) for n in sv.nphi]
+    times = {"forward": [], "backward": []}
+    for rep in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sv.forward()
+        ctx.synchronize()
+        times["forward"].append(time.perf_counter() - t0)
+        phi = [chunk_get(ctx, p, n) for p, n in zip(sv.phi_ptrs(), sv.nphi)]
+        loss = sum(float(np.sum(((v - tg) ** 2)[zm])) for v, tg in zip(phi, target))
+        for b, v, tg in zip(wb, phi, target):
+            chunk_set(ctx, b.ptr.value, np.where(zm, 2.0 * (v - tg), 0.0))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sv.backward([b.ptr.value for b in wb])
+        ctx.synchronize()
+        times["backward"].append(time.perf_counter() - t0)
+    g = chunk_get(ctx, sv.param_ptrs("grad_sigt")[0], sv.nparam[0])
+    for b in bt + bs + wb:
+        b.free()
+    sv.close()
+    ctx.close()
+    tf, tb = min(times["forward"]), min(times["backward"])
+    return {"workload": "32^3 zones, 32 groups, 96 directions GL 8x12, P2 (9 moments), %s, 10 iterations from "
+                        "psi=0, phi checkpoints every 2; parameters: sigma_t and sigma_s per zone and group; loss on "
+                        "a seeded 4^3 detector box" % layout,
+            "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "adjoint_forward_cost": tb / tf,
+            "loss": loss, "grad_sigt_norm": float(np.linalg.norm(g))}
+
+
 def run_cpu_baseline():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -114,6 +186,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--layout", default="DGZ")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -167,6 +240,8 @@ def main():
         "breakdown_ms": {n: timers[n][0] / nb * 1e3 for n in timers},
         "setup_s": setup_s,
     }
+    if not args.no_adjoint:
+        out["adjoint"] = run_adjoint(args.layout)
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out))
