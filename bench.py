@@ -8,8 +8,9 @@ pipelined sweep and the population, over the workload
     Legendre 8 polar x 16 azimuthal, 8 direction sets), P4 scattering
     (25 moments), FP64, DGZ nesting, seeded 3-material box geometry.
 
-A second leg measures the metric's adjoint/forward cost on the
-differentiable solve (run_adjoint).  Prints one JSON line.  `value` is the
+Further legs measure the metric's adjoint/forward cost on the
+differentiable solve (run_adjoint) and the batched persistent launch of 48
+small problems (run_batched).  Prints one JSON line.  `value` is the
 reference figure of merit
 (src/kripke.cpp:495-505): unknowns (D*G*Z) per second per iteration, with all
 fields resident in HBM when the timed region starts.
@@ -160,6 +161,69 @@ def run_adjoint(layout):
             "loss": loss, "grad_sigt_norm": float(np.linalg.norm(g))}
 
 
+BATCH_PROBLEMS, BATCH_ITERATIONS = 48, 15
+
+
+def batch_configs(n=BATCH_PROBLEMS, base_seed=20261202):
+    """The batched workload: n independent problems, per-problem seed
+    base + index, zones 16 +- 4 per axis, G in {8,16,32,64}, D in
+    {32,64,96,128} (GL, 8 polar angles), Legendre order 1..4, DGZ."""
+    import numpy as np
+    rng = np.random.default_rng(base_seed)
+    out = []
+    for i in range(n):
+        zones = tuple(int(v) for v in np.clip(16 + rng.integers(-4, 5, size=3), 8, 40))
+        out.append(dict(zones=zones, groups=int(rng.choice([8, 16, 32, 64])),
+                        quad=(8, int(rng.choice([4, 8, 12, 16]))), legendre=int(rng.integers(1, 5)), layout="DGZ",
+                        dset=8, gset=1, zset=(1, 1, 1), geometry=1, seed=base_seed + i))
+    return out
+
+
+def run_batched():
+    """48 independent problems x 15 iterations: one hipGraph launch (each
+    problem's sweep shape autotuned, cached per shape) against the same
+    iterations issued problem by problem, one call at a time."""
+    import numpy as np
+    import kripke_amd as ka
+    ctx = ka.Context(0)
+    cfgs = batch_configs()
+    hps = [ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs]
+    t0 = time.perf_counter()
+    for hp in hps:
+        hp.kernels().autotune()
+    tune_s = time.perf_counter() - t0
+    unknowns = sum(c["zones"][0] * c["zones"][1] * c["zones"][2] * c["groups"] * c["quad"][0] * c["quad"][1]
+                   for c in cfgs)
+    # problem by problem
+    for hp in hps:
+        hp.init_psi()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    seq = np.array([[hp.iterate(1) for _ in range(BATCH_ITERATIONS)] for hp in hps])
+    seq_s = time.perf_counter() - t0
+    # one graph launch (the first call captures; time a replay from psi = 0)
+    b = ka.Batch(ctx, hps)
+    for hp in hps:
+        hp.init_psi()
+    b.iterate(BATCH_ITERATIONS)
+    for hp in hps:
+        hp.init_psi()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    pops = b.iterate(BATCH_ITERATIONS)
+    graph_s = time.perf_counter() - t0
+    same = bool(np.array_equal(pops, seq))
+    b.close()
+    for hp in hps:
+        hp.close()
+    ctx.close()
+    return {"workload": "%d problems (zones 16+-4 per axis, G in {8,16,32,64}, D in {32,64,96,128}, P1-P4, DGZ), "
+                        "%d iterations each" % (BATCH_PROBLEMS, BATCH_ITERATIONS),
+            "unknowns_per_iteration": unknowns, "graph_ms": graph_s * 1e3, "per_problem_ms": seq_s * 1e3,
+            "speedup_vs_per_problem": seq_s / graph_s, "graph_unknowns_per_s": unknowns * BATCH_ITERATIONS / graph_s,
+            "populations_equal_per_problem_runs": same, "autotune_s": tune_s}
+
+
 def run_cpu_baseline():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -187,6 +251,7 @@ def main():
     ap.add_argument("--layout", default="DGZ")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -242,6 +307,8 @@ def main():
     }
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint(args.layout)
+    if not args.no_batch:
+        out["batched"] = run_batched()
     if not args.no_cpu:
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out))

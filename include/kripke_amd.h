@@ -186,6 +186,35 @@ int kr_sweep_solver_adjoint(kr_problem *prob);
 int kr_grad_sigt(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
 int kr_grad_sigs_zonal(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
 
+/* ---- autotuner --------------------------------------------------------- */
+/* No counterpart in the reference.  Times the pipelined sweep (the dominant
+ * kernel) over its shape candidates -- zones per lane of the row-scan kernel
+ * or the systolic kernel's patch size, times three work-unit sizes -- on
+ * this problem's own fields, installs the fastest, and caches the choice on
+ * the context for every later problem of the same shape.  It runs the sweep,
+ * so psi and the planes are overwritten: call it before the solve (or before
+ * kr_batch_create).  choice (may be NULL) receives {kernel kind, zones per
+ * lane or patch columns, unit size in percent of the default}. */
+int kr_problem_autotune(kr_problem *prob, int32_t *choice);
+
+/* ---- batched persistent launch ------------------------------------------ */
+/* No counterpart in the reference: many independent problems (each a
+ * registered kr_problem on one context) advanced together.  kr_batch_iterate
+ * runs niter source iterations of every problem -- the on-device pipeline:
+ * LTimes, scattering, source(strength), LPlusTimes, SweepSolver with the
+ * population -- as ONE hipGraph launch with one branch per problem, and
+ * returns the populations as populations[problem*niter + iteration].  Each
+ * problem's results equal those of its own iterations run one call at a time
+ * (same kernels in the same order).  The graph is captured on the first call
+ * and re-captured when niter changes; the problems' fields must not be
+ * re-registered while the batch lives.  Blocks until the values are on the
+ * host. */
+typedef struct kr_batch kr_batch;
+int kr_batch_create(kr_context *ctx, kr_problem *const *problems, const double *source_strength, int32_t n,
+                    kr_batch **out);
+int kr_batch_destroy(kr_batch *batch);
+int kr_batch_iterate(kr_batch *batch, int32_t niter, double *populations);
+
 /* ---- per-entry-point timing (hipEvents on the context stream) ---------- */
 /* Names: LTimes, LPlusTimes, Scattering, Source, SweepSubdomain,
  * SweepSolver, Population, kConst.  kr_problem_timer synchronizes. */

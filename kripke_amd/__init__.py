@@ -3,5 +3,5 @@
 The product is libkripke_amd.so (HIP kernels for gfx950 + C++ host mirror)
 behind the C-ABI in include/kripke_amd.h; this package only binds it.
 """
-from .capi import (LAYOUTS, Context, HostProblem, InputVars, KernelProblem, KripkeError, device_count,  # noqa: F401
+from .capi import (LAYOUTS, Batch, Context, HostProblem, InputVars, KernelProblem, KripkeError, device_count,  # noqa: F401
                    input_vars, lib)

@@ -108,6 +108,10 @@ SIGNATURES = [
     ("kr_sweep_solver_adjoint", ctypes.c_int, [P]),
     ("kr_grad_sigt", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_grad_sigs_zonal", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
+    ("kr_problem_autotune", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32)]),
+    ("kr_batch_create", ctypes.c_int, [P, ctypes.POINTER(P), DP, ctypes.c_int32, ctypes.POINTER(P)]),
+    ("kr_batch_destroy", ctypes.c_int, [P]),
+    ("kr_batch_iterate", ctypes.c_int, [P, ctypes.c_int32, DP]),
     ("kr_problem_set_timing", ctypes.c_int, [P, ctypes.c_int32]),
     ("kr_problem_timer", ctypes.c_int, [P, ctypes.c_char_p, DP, ctypes.POINTER(ctypes.c_int64)]),
     ("kr_problem_timer_reset", ctypes.c_int, [P]),
@@ -250,6 +254,13 @@ class KernelProblem:
         arr = (P * len(ptrs))(*ptrs)
         check(lib().kr_grad_sigs_zonal(self.h, adj.h, arr, 1 if accumulate else 0), "kr_grad_sigs_zonal")
 
+    def autotune(self):
+        """Pick the fastest sweep shape for this problem (overwrites psi and the
+        planes); returns (kernel kind, zones per lane or patch columns, unit %)."""
+        c = (ctypes.c_int32 * 3)()
+        check(lib().kr_problem_autotune(self.h, c), "kr_problem_autotune")
+        return tuple(c)
+
     def set_timing(self, on=True):
         check(lib().kr_problem_set_timing(self.h, 1 if on else 0), "kr_problem_set_timing")
 
@@ -260,6 +271,37 @@ class KernelProblem:
 
     def timer_reset(self):
         check(lib().kr_problem_timer_reset(self.h), "kr_problem_timer_reset")
+
+
+class Batch:
+    """kr_batch: independent host problems on one context advanced by one
+    hipGraph launch per call (include/kripke_amd.h)."""
+
+    def __init__(self, ctx, host_problems):
+        self.ctx, self.problems = ctx, list(host_problems)
+        ks = [hp.kernels() for hp in self.problems]
+        arr = (P * len(ks))(*[k.h for k in ks])
+        st = (ctypes.c_double * len(ks))(*[hp.inp.source_strength for hp in self.problems])
+        self.h = P()
+        check(lib().kr_batch_create(ctx.h, arr, st, len(ks), ctypes.byref(self.h)), "kr_batch_create")
+        ctx._children.add(self)  # destroyed with the context
+
+    def iterate(self, niter):
+        """Populations, shape (problems, niter)."""
+        out = np.zeros(len(self.problems) * niter, dtype=np.float64)
+        check(lib().kr_batch_iterate(self.h, niter, out.ctypes.data_as(DP)), "kr_batch_iterate")
+        return out.reshape(len(self.problems), niter)
+
+    def close(self):
+        if self.h:
+            lib().kr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HostProblem:

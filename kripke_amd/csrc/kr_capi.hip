@@ -15,9 +15,17 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 
 using namespace kr;
 
+// Autotuner result for one problem shape (kr_problem_autotune).
+struct TuneChoice {
+  int v_max = 0, cap_div = 1;
+  int r_scale_pct = 100;  // pair groups per work unit, percent of the default
+  double ms = 0.0;
+};
+
 struct kr_context {
   int device;
   hipStream_t stream;
+  std::map<std::vector<int64_t>, TuneChoice> tuned;  // shape key -> choice
 };
 
 struct TimedCall {
@@ -42,6 +50,9 @@ struct kr_problem {
   SweepPlanes *d_planes_ref = nullptr, *d_planes_pipe = nullptr;
   // sweep schedules
   SweepShape shape_ref{}, shape_pipe{};
+  bool aligned16 = true;
+  TuneChoice tune;
+  int64_t sched_version = 0;  // bumped when the pipelined schedule is replaced (captured graphs go stale)
   SweepUnit *d_units_ref = nullptr;            // all sdoms, grouped by sdom
   std::vector<int32_t> ref_unit_begin, ref_unit_count;
   SweepUnit *d_units_pipe = nullptr;           // all sdoms, grouped by level
@@ -520,6 +531,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     const void *ptrs[] = {H.psi, H.rhs, H.sig, H.plane[0], H.plane[1], H.plane[2]};
     for (const void *q : ptrs) aligned16 = aligned16 && ((uintptr_t)q % 16 == 0);
   }
+  p->aligned16 = aligned16;
   p->shape_ref = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 1, aligned16);
   p->shape_pipe = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 0, aligned16);
   const int32_t pgs = (int32_t)(((int64_t)nd * ng) / p->shape_pipe.Q);
@@ -849,6 +861,300 @@ int kr_grad_sigs_zonal(kr_problem *fwd, kr_problem *adj, double *const *grad, in
   int64_t sg, sz;
   sigt_layout(fwd, &sg, &sz);
   return launch_grad_sigs_zonal(launch_ctx(fwd), adj->d_pr, (double *const *)dev, sg, sz, accumulate ? 1 : 0);
+}
+
+// ---- autotuner ---------------------------------------------------------------
+
+// Pipelined-sweep schedule for a shape and unit-size scale: units grouped
+// by dependency level.
+static int build_pipe_schedule(kr_problem *p, const SweepShape &sh, int r_scale_pct, std::vector<SweepUnit> &units,
+                               std::vector<int32_t> &lbeg, std::vector<int32_t> &lcnt) {
+  std::vector<int32_t> all(p->nsdom);
+  for (int s = 0; s < p->nsdom; ++s) all[s] = s;
+  std::vector<std::vector<int32_t>> levels;
+  int rc = sweep_levels(p, all, levels);
+  if (rc) return rc;
+  const int32_t pgs = (int32_t)(((int64_t)p->h_sd[0].nd * p->h_sd[0].ng) / sh.Q);
+  units.clear();
+  lbeg.clear();
+  lcnt.clear();
+  for (auto &lv : levels) {
+    int64_t R = (int64_t)pick_R((int64_t)pgs * lv.size(), sh) * r_scale_pct / 100;
+    R = std::max<int64_t>(1, std::min<int64_t>(R, 64));
+    lbeg.push_back((int32_t)units.size());
+    for (int32_t s : lv) append_units(units, s, pgs, (int32_t)R);
+    lcnt.push_back((int32_t)units.size() - lbeg.back());
+  }
+  return check_unit_spans(p, units, sh);
+}
+
+static std::vector<int64_t> tune_key(const kr_problem *p) {
+  const DevSdom &S = p->h_sd[0];
+  return {p->nesting, p->nsdom, S.nd, S.ng, S.ni, S.nj, S.nk, p->aligned16 ? 1 : 0};
+}
+
+static int install_choice(kr_problem *p, const TuneChoice &c) {
+  const DevSdom &S = p->h_sd[0];
+  SweepTune t;
+  t.v_max = c.v_max;
+  t.cap_div = c.cap_div;
+  const SweepShape sh = choose_sweep_shape(p->nesting, S.nd, S.ng, S.ni, S.nj, S.nk, 0, p->aligned16, t);
+  if (sh.kind == 2) return fail(KR_ERR_UNSUPPORTED, "autotune: shape does not fit");
+  std::vector<SweepUnit> u;
+  std::vector<int32_t> lb, lc;
+  int rc = build_pipe_schedule(p, sh, c.r_scale_pct, u, lb, lc);
+  if (rc) return rc;
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  SweepUnit *d = nullptr;
+  if ((rc = upload(p->ctx, &d, u))) return rc;
+  if (p->d_units_pipe) KR_HIP_CHECK(hipFree(p->d_units_pipe));
+  p->d_units_pipe = d;
+  p->pipe_level_begin = lb;
+  p->pipe_level_count = lc;
+  p->shape_pipe = sh;
+  p->tune = c;
+  ++p->sched_version;
+  return KR_OK;
+}
+
+int kr_problem_autotune(kr_problem *p, int32_t *choice) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_problem_autotune: null problem");
+  const std::vector<int64_t> key = tune_key(p);
+  auto it = p->ctx->tuned.find(key);
+  TuneChoice best;
+  if (it != p->ctx->tuned.end()) {
+    best = it->second;
+  } else {
+    std::vector<TuneChoice> cand;
+    const bool zfast = p->nesting == KR_DGZ || p->nesting == KR_GDZ;
+    for (int pct : {50, 100, 200}) {
+      if (zfast) {
+        for (int v : {0, 1}) {
+          TuneChoice c;
+          c.v_max = v;
+          c.r_scale_pct = pct;
+          cand.push_back(c);
+        }
+      } else {
+        for (int cd : {1, 2}) {
+          TuneChoice c;
+          c.cap_div = cd;
+          c.r_scale_pct = pct;
+          cand.push_back(c);
+        }
+      }
+    }
+    hipEvent_t e0, e1;
+    KR_HIP_CHECK(hipEventCreate(&e0));
+    KR_HIP_CHECK(hipEventCreate(&e1));
+    double best_ms = 1e300;
+    int rc = KR_OK;
+    for (TuneChoice &c : cand) {
+      if ((rc = install_choice(p, c))) {
+        if (rc == KR_ERR_UNSUPPORTED) continue;
+        break;
+      }
+      double ms = 1e300;
+      for (int rep = 0; rep < 3 && !rc; ++rep) {  // first run warms up; keep the best of the rest
+        hipEventRecord(e0, p->ctx->stream);
+        for (size_t lv = 0; lv < p->pipe_level_begin.size() && !rc; ++lv)
+          rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
+                            p->pipe_level_count[lv], p->shape_pipe);
+        hipEventRecord(e1, p->ctx->stream);
+        hipEventSynchronize(e1);
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        if (rep > 0) ms = std::min(ms, (double)t);
+      }
+      c.ms = ms;
+      if (!rc && ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    p->ctx->tuned[key] = best;
+  }
+  int rc = install_choice(p, best);
+  if (rc) return rc;
+  if (choice) {
+    choice[0] = p->shape_pipe.kind;
+    choice[1] = p->shape_pipe.kind == 1 ? p->shape_pipe.V : p->shape_pipe.PJ * p->shape_pipe.PK;
+    choice[2] = best.r_scale_pct;
+  }
+  return KR_OK;
+}
+
+// ---- batched persistent launch ----------------------------------------------
+
+// Device buffer of one population partial per pipelined sweep unit.
+static int ensure_pop_units(kr_problem *p, int64_t *n) {
+  *n = p->pipe_level_begin.empty() ? 0 : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
+  if (*n > p->pop_units_cap) {
+    if (p->d_pop_units) KR_HIP_CHECK(hipFree(p->d_pop_units));
+    p->d_pop_units = nullptr;
+    KR_HIP_CHECK(hipMalloc((void **)&p->d_pop_units, std::max<int64_t>(*n, 1) * sizeof(double)));
+    p->pop_units_cap = *n;
+  }
+  return KR_OK;
+}
+
+// One source iteration of the on-device pipeline (the mode-1 sequence of the
+// host mirror's SteadyStateIteration) enqueued on stream st, with the
+// population written to pop_dev.  Only kernel launches: capturable.
+static int iterate_on(kr_problem *p, hipStream_t st, double strength, double *pop_dev) {
+  LaunchCtx L = launch_ctx(p);
+  L.stream = st;
+  int rc;
+  if ((rc = launch_ltimes(L, 0)) || (rc = launch_scattering(L, p->pr_gset.data(), p->pr_r.data(), 0)) ||
+      (rc = launch_source(L, strength)) || (rc = launch_lplustimes(L, 0)))
+    return rc;
+  if ((rc = launch_sweep_coef(st, p->d_sd, 0, p->nsdom))) return rc;
+  for (size_t lv = 0; lv < p->pipe_level_begin.size(); ++lv) {
+    rc = launch_sweep(st, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
+                      p->pipe_level_count[lv], p->shape_pipe, p->d_pop_units + p->pipe_level_begin[lv]);
+    if (rc) return rc;
+  }
+  const int64_t n = p->pipe_level_begin.empty() ? 0
+                                                : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
+  return launch_sum(st, p->d_pop_units, n, pop_dev);
+}
+
+struct kr_batch {
+  kr_context *ctx = nullptr;
+  std::vector<kr_problem *> probs;
+  std::vector<double> strength;
+  std::vector<hipStream_t> streams;  // one branch per problem while capturing
+  std::vector<hipEvent_t> joins;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int32_t graph_niter = -1;
+  std::vector<int64_t> versions;     // the problems' schedule versions at capture
+  double *d_pop = nullptr;           // [problem][iteration]
+  int64_t pop_cap = 0;
+};
+
+int kr_batch_destroy(kr_batch *b) {
+  if (!b) return KR_OK;
+  if (b->ctx) hipStreamSynchronize(b->ctx->stream);
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  for (hipStream_t s : b->streams) hipStreamDestroy(s);
+  for (hipEvent_t e : b->joins) hipEventDestroy(e);
+  if (b->fork) hipEventDestroy(b->fork);
+  if (b->d_pop) hipFree(b->d_pop);
+  delete b;
+  return KR_OK;
+}
+
+int kr_batch_create(kr_context *ctx, kr_problem *const *problems, const double *source_strength, int32_t n,
+                    kr_batch **out) {
+  if (!ctx || !problems || !source_strength || n <= 0 || !out)
+    return fail(KR_ERR_INVALID, "kr_batch_create: bad argument");
+  *out = nullptr;
+  auto *b = new kr_batch();
+  b->ctx = ctx;
+  for (int32_t i = 0; i < n; ++i) {
+    if (!problems[i] || problems[i]->ctx != ctx) {
+      kr_batch_destroy(b);
+      return fail(KR_ERR_INVALID, "kr_batch_create: every problem must be registered on this context");
+    }
+    for (int32_t j = 0; j < i; ++j)
+      if (problems[j] == problems[i]) {
+        kr_batch_destroy(b);
+        return fail(KR_ERR_INVALID, "kr_batch_create: a problem appears twice");
+      }
+    b->probs.push_back(problems[i]);
+    b->strength.push_back(source_strength[i]);
+  }
+  b->streams.resize(n, nullptr);
+  b->joins.resize(n, nullptr);
+  for (int32_t i = 0; i < n; ++i) {
+    if (hipStreamCreateWithFlags(&b->streams[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming) != hipSuccess) {
+      kr_batch_destroy(b);
+      return fail(KR_ERR_HIP, "kr_batch_create: stream/event creation failed");
+    }
+  }
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+    kr_batch_destroy(b);
+    return fail(KR_ERR_HIP, "kr_batch_create: event creation failed");
+  }
+  *out = b;
+  return KR_OK;
+}
+
+// Builds the graph: a fork from the context stream, one branch per problem
+// holding niter iterations, a join back.
+static int batch_capture(kr_batch *b, int32_t niter) {
+  if (b->exec) {
+    KR_HIP_CHECK(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    KR_HIP_CHECK(hipGraphDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  const int64_t need = (int64_t)b->probs.size() * niter;
+  if (need > b->pop_cap) {
+    if (b->d_pop) KR_HIP_CHECK(hipFree(b->d_pop));
+    b->d_pop = nullptr;
+    KR_HIP_CHECK(hipMalloc((void **)&b->d_pop, need * sizeof(double)));
+    b->pop_cap = need;
+  }
+  for (kr_problem *p : b->probs) {
+    int64_t nu;
+    int rc = ensure_pop_units(p, &nu);
+    if (rc) return rc;
+  }
+  hipStream_t s0 = b->ctx->stream;
+  KR_HIP_CHECK(hipStreamSynchronize(s0));
+  KR_HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  int rc = KR_OK;
+  if (hipEventRecord(b->fork, s0) != hipSuccess) rc = fail(KR_ERR_HIP, "batch capture: fork");
+  for (size_t i = 0; !rc && i < b->probs.size(); ++i) {
+    if (hipStreamWaitEvent(b->streams[i], b->fork, 0) != hipSuccess) {
+      rc = fail(KR_ERR_HIP, "batch capture: branch");
+      break;
+    }
+    for (int32_t it = 0; !rc && it < niter; ++it)
+      rc = iterate_on(b->probs[i], b->streams[i], b->strength[i], b->d_pop + (int64_t)i * niter + it);
+    if (!rc && (hipEventRecord(b->joins[i], b->streams[i]) != hipSuccess ||
+                hipStreamWaitEvent(s0, b->joins[i], 0) != hipSuccess))
+      rc = fail(KR_ERR_HIP, "batch capture: join");
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s0, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(KR_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(ec));
+  b->graph = g;
+  KR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_niter = niter;
+  b->versions.clear();
+  for (kr_problem *p : b->probs) b->versions.push_back(p->sched_version);
+  return KR_OK;
+}
+
+int kr_batch_iterate(kr_batch *b, int32_t niter, double *populations) {
+  if (!b || niter <= 0 || !populations) return fail(KR_ERR_INVALID, "kr_batch_iterate: bad argument");
+  bool stale = niter != b->graph_niter;
+  for (size_t i = 0; !stale && i < b->probs.size(); ++i) stale = b->versions[i] != b->probs[i]->sched_version;
+  if (stale) {  // first call, another iteration count, or a problem re-tuned since the capture
+    int rc = batch_capture(b, niter);
+    if (rc) return rc;
+  }
+  hipStream_t s0 = b->ctx->stream;
+  KR_HIP_CHECK(hipGraphLaunch(b->exec, s0));
+  KR_HIP_CHECK(hipMemcpyAsync(populations, b->d_pop, sizeof(double) * b->probs.size() * niter,
+                              hipMemcpyDeviceToHost, s0));
+  KR_HIP_CHECK(hipStreamSynchronize(s0));
+  return KR_OK;
 }
 
 int kr_population(kr_problem *p, double *out) {

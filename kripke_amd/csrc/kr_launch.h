@@ -63,8 +63,15 @@ int launch_sum(hipStream_t st, const double *partial, int64_t n, double *out);
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).
 int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // Shape choice for a subdomain extent and nesting (pair lanes, patch).
+// Tuning knobs of the shape choice (the autotuner's search space):
+// v_max caps the row-scan kernel's zones per lane (0: no cap); cap_div
+// shrinks the systolic kernel's largest patch by that factor.
+struct SweepTune {
+  int v_max = 0;
+  int cap_div = 1;
+};
 // aligned16: every psi/rhs/sigma_t/plane chunk starts on a 16-byte boundary.
 SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,
-                              bool aligned16);
+                              bool aligned16, const SweepTune &tune = SweepTune());
 
 }  // namespace kr

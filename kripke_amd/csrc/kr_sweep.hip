@@ -397,9 +397,10 @@ static int largest_divisor_le(int n, int cap) {
 // Row-scan shape for zone-fastest nestings: V zones per lane (V > 1 needs
 // 16-byte aligned rows), W >= ni/V lanes per row group, one row group per
 // k-row.  Returns false when the subdomain does not fit one workgroup.
-static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, bool aligned16) {
+static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, bool aligned16,
+                      int v_max) {
   int want = ni % 2 == 0 ? 2 : 1;
-  if (const char *e = getenv("KRIPKE_AMD_ROW_ZONES")) want = atoi(e);  // tuning override: 1 or 2
+  if (v_max > 0) want = std::min(want, v_max);
   const int64_t nz = (int64_t)ni * nj * nk;
   const int64_t maxface = std::max({(int64_t)nj * nk, (int64_t)ni * nk, (int64_t)ni * nj});
   // every stream is a buffer resource based at the unit's first item and
@@ -427,7 +428,7 @@ static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_
 }
 
 SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,
-                              bool aligned16) {
+                              bool aligned16, const SweepTune &tune) {
   SweepShape sh{};
   sh.ref_mode = ref_mode;
   sh.nd = nd;
@@ -438,7 +439,7 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   // inner pair index = the faster of (d, g) in psi's memory order
   sh.inner_is_g = (nesting == KR_DGZ || nesting == KR_DZG || nesting == KR_ZDG) ? 1 : 0;
   const int inner = sh.inner_is_g ? ng : nd;
-  if (zfast && row_shape(sh, nd, ng, ni, nj, nk, aligned16)) return sh;
+  if (zfast && row_shape(sh, nd, ng, ni, nj, nk, aligned16, tune.v_max)) return sh;
   if (zfast) {
     sh.Q = 1;
   } else {
@@ -448,7 +449,7 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   }
   // LDS: j/k face exchange [4][threads] and the coefficient records [nd][ni+nj+nk+1]
   const size_t coef_lds = (size_t)nd * (ni + nj + nk + 1) * sizeof(double);
-  for (int cap = 1024 / sh.Q; cap >= 1; cap /= 2) {
+  for (int cap = 1024 / sh.Q / std::max(1, tune.cap_div); cap >= 1; cap /= 2) {
     // patch: as many (j,k) columns as fit, subject to the pipeline distances
     // (a patch's downstream neighbour prefetches a face COL_RING steps before
     // it uses it, and must do so at least one step after the face is written).

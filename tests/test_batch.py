@@ -1,0 +1,94 @@
+"""Batched persistent launch (kr_batch): every problem of a batch gives the
+same populations and phi as its own iterations run one call at a time, bit
+for bit, across several graph replays with eager work in between and a
+re-capture when the iteration count changes."""
+import numpy as np
+import pytest
+
+import kripke_amd as ka
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_problems(n, seed0=100):
+    """Small problems of varied shape, nesting, groups, directions and
+    scattering order (the batched configuration, scaled down)."""
+    rng = np.random.default_rng(seed0)
+    layouts = ["DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD"]
+    out = []
+    for i in range(n):
+        zones = tuple(int(v) for v in np.clip(8 + rng.integers(-2, 5, size=3), 6, 12))
+        out.append(dict(zones=zones, groups=int(rng.choice([4, 8])), quad=(4, int(rng.choice([2, 4]))),
+                        legendre=int(rng.integers(1, 4)), layout=layouts[i % 6], dset=8, gset=1,
+                        zset=(2, 1, 1) if (i % 2 and zones[0] % 2 == 0) else (1, 1, 1), geometry=1,
+                        seed=seed0 + i))
+    return out
+
+
+def test_batch_matches_single_problem_runs(ctx):
+    cfgs = mixed_problems(6)
+    ref_pops, ref_phi = [], []
+    for c in cfgs:
+        hp = ka.HostProblem(ka.input_vars(**c), ctx)
+        hp.init_psi()
+        ref_pops.append([hp.iterate(1) for _ in range(5)])
+        ref_phi.append([hp.get("phi", k) for k in range(hp.num_pr)])
+        hp.close()
+    hps = [ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs]
+    for hp in hps:
+        hp.init_psi()
+    b = ka.Batch(ctx, hps)
+    p1 = b.iterate(2)
+    hps[0].kernels().population()          # eager work between replays
+    p2 = b.iterate(2)
+    ctx.synchronize()
+    p3 = b.iterate(1)                       # re-capture with another iteration count
+    got = np.concatenate([p1, p2, p3], axis=1)
+    for i in range(len(cfgs)):
+        assert np.array_equal(got[i], np.array(ref_pops[i])), "problem %d: %s vs %s" % (i, got[i], ref_pops[i])
+        for k in range(hps[i].num_pr):
+            assert np.array_equal(hps[i].get("phi", k), ref_phi[i][k])
+    b.close()
+    for hp in hps:
+        hp.close()
+
+
+def test_batch_rejects_bad_input(ctx):
+    hp = ka.HostProblem(ka.input_vars(**mixed_problems(1)[0]), ctx)
+    with pytest.raises(ka.KripkeError):
+        ka.Batch(ctx, [hp, hp])
+    hp.close()
+
+
+@pytest.mark.parametrize("layout", ["DGZ", "DZG"])
+def test_autotune_keeps_results(ctx, layout):
+    """The tuned sweep schedule changes blocking only: populations and phi
+    agree with the untuned problem (scan re-association at most), and a
+    second problem of the same shape reuses the cached choice."""
+    cfg = dict(zones=(16, 16, 16), groups=8, quad=(4, 4), legendre=2, layout=layout, dset=8, gset=1,
+               zset=(2, 2, 2), geometry=1, seed=7)
+    a = ka.HostProblem(ka.input_vars(**cfg), ctx)
+    b = ka.HostProblem(ka.input_vars(**cfg), ctx)
+    choice = b.kernels().autotune()
+    assert choice[2] in (50, 100, 200)
+    assert b.kernels().autotune() == choice  # cached
+    c = ka.HostProblem(ka.input_vars(**cfg), ctx)
+    assert c.kernels().autotune() == choice
+    for hp in (a, b):
+        hp.init_psi()
+    for _ in range(3):
+        pa, pb = a.iterate(1), b.iterate(1)
+        assert abs(pa - pb) <= 1e-13 * abs(pa)
+    for k in range(a.num_pr):
+        x, y = a.get("phi", k), b.get("phi", k)
+        assert np.max(np.abs(x - y)) <= 1e-12 * np.max(np.abs(x))
+    # a batch built after tuning, and re-captured after re-tuning, still matches
+    bt = ka.Batch(ctx, [c])
+    c.init_psi()
+    pops = bt.iterate(3)[0]
+    a.init_psi()
+    ref = [a.iterate(1) for _ in range(3)]
+    assert np.allclose(pops, ref, rtol=1e-13, atol=0)
+    bt.close()
+    for hp in (a, b, c):
+        hp.close()
