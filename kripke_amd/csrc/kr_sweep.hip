@@ -411,17 +411,25 @@ static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_
     if (V > want || ni % V != 0 || (V > 1 && !aligned16)) continue;
     int W = 4;
     while (W * V < ni) W *= 2;
-    const size_t lds = (2 * (size_t)W * nk * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
-    if (W <= 64 && W * nk <= 1024 && lds <= 65536) {
-      sh.kind = 1;
-      sh.RG = W;
-      sh.V = V;
-      sh.Q = 1;
-      sh.PJ = nj;
-      sh.PK = nk;
-      sh.NPJ = 1;
-      sh.NPK = 1;
-      return true;
+    if (W > 64) continue;
+    // k-rows per block: all of them, or a divisor of nk; between k-patches the
+    // k-face goes through the own plane and must be written (by the patch's
+    // top row group, PK-1 steps into a row) before the next patch prefetches
+    // it (4 steps ahead): nj >= PK + 4
+    for (int PK = largest_divisor_le(nk, 1024 / W); PK >= 1; PK = largest_divisor_le(nk, PK - 1)) {
+      const size_t lds = (2 * (size_t)W * PK * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
+      if ((PK == nk || nj >= PK + 4) && lds <= 65536) {
+        sh.kind = 1;
+        sh.RG = W;
+        sh.V = V;
+        sh.Q = 1;
+        sh.PJ = nj;
+        sh.PK = PK;
+        sh.NPJ = 1;
+        sh.NPK = nk / PK;
+        return true;
+      }
+      if (PK == 1) break;
     }
   }
   return false;

@@ -137,23 +137,28 @@ struct RowArgs {
   const double *in[3];
   double *out[3];
   const double *w, *vol;  // quadrature weights, zone volumes (population)
+  double *own2;           // the subdomain's own k-plane: faces between k-patches
   double *pop;            // population partial of this block, or nullptr
   int32_t nd, ng, ni, nj, nk, jd, kd;
   int32_t pg_begin, R, inner_is_g;
 };
 
-// Byte offsets of one row group's current row, stepped with additions.
-// Before the first row the cursor rests on row 0 of the first item; past the
-// last item it keeps the last item, so every offset stays in bounds.
+// Byte offsets of one row group's current row, stepped with additions.  A
+// unit walks, per item, its k-patches (PK k-rows each, one per row group) and
+// per patch the nj rows.  Offsets split into an item part, a k part (the row
+// group's k-row in the current patch) and a j part.  Before the first row the
+// cursor rests on row 0 of the first patch of the first item; past the last
+// item it keeps the last item, so every offset stays in bounds.
 struct RowCursor {
   int32_t pos;          // rows since the unit's first row (negative: not started)
-  int32_t jj;           // row within the item, sweep order
+  int32_t jj, kp;       // row within the patch (sweep order), k-patch
   uint32_t j8, jrow8;   // 8*j, 8*j*ni
-  uint32_t z8;          // 8*(item*nz + lane's zone in row 0)
-  uint32_t s8;          // 8*(g*nz + lane's zone in row 0)
-  uint32_t f8[3];       // x: 8*(item*nf0 + k*nj), j: 8*(item*nf1 + k*ni + lane), k: 8*(item*nf2 + lane)
+  uint32_t zi8, si8;    // 8*item*nz (relative to the unit's first item), 8*g*nz
+  uint32_t fi8[3];      // 8*item*nf per axis (relative)
+  uint32_t kz8;         // 8*(k*nj*ni + lane's first zone)
+  uint32_t kx8, ky8;    // 8*k*nj (i-plane), 8*(k*ni + lane's first zone) (j-plane)
   int32_t d, g;
-  int32_t bco;          // d*(ni+nj+nk): the item's coefficient record in LDS
+  int32_t bco;          // d*(ni+nj+nk+1): the item's coefficient record in LDS
 };
 
 struct RowConst {
@@ -161,13 +166,16 @@ struct RowConst {
   int32_t L;
   uint32_t j08, jrow08;            // first row in sweep order
   int32_t jstep8, jrowstep8;       // per-row increments
-  int32_t rows;                    // R*nj
+  int32_t NPK;                     // k-patches per item
+  int32_t dkz8, dkx8, dky8;        // offset steps from one k-patch to the next
+  int32_t rows;                    // R*NPK*nj
 };
 
 __device__ __forceinline__ void cursor_init(RowCursor &c, const RowArgs &P, const RowConst &K, int pos,
-                                            uint32_t zk8, uint32_t xk8, uint32_t yk8, uint32_t mb8) {
+                                            uint32_t kz8, uint32_t kx8, uint32_t ky8) {
   c.pos = pos;
   c.jj = 0;
+  c.kp = 0;
   c.j8 = K.j08;
   c.jrow8 = K.jrow08;
   const int p = P.pg_begin;
@@ -180,11 +188,13 @@ __device__ __forceinline__ void cursor_init(RowCursor &c, const RowArgs &P, cons
   }
   // psi/rhs and plane offsets are relative to the unit's first item (the
   // resources are based there), so they stay below 2 GiB for any chunk size
-  c.z8 = zk8;
-  c.s8 = (uint32_t)c.g * K.nz8 + zk8;
-  c.f8[0] = xk8;
-  c.f8[1] = yk8;
-  c.f8[2] = mb8;
+  c.zi8 = 0;
+  c.si8 = (uint32_t)c.g * K.nz8;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) c.fi8[x] = 0;
+  c.kz8 = kz8;
+  c.kx8 = kx8;
+  c.ky8 = ky8;
   c.bco = c.d * K.L;
 }
 
@@ -199,23 +209,33 @@ __device__ __forceinline__ void cursor_step(RowCursor &c, const RowArgs &P, cons
   c.j8 = K.j08;
   c.jrow8 = K.jrow08;
   if (c.pos >= K.rows) return;  // past the last item: stay on it
-  c.z8 += K.nz8;
+  if (++c.kp < K.NPK) {         // next k-patch of the same item
+    c.kz8 += K.dkz8;
+    c.kx8 += K.dkx8;
+    c.ky8 += K.dky8;
+    return;
+  }
+  c.kp = 0;                     // first k-patch of the next item
+  c.kz8 -= (K.NPK - 1) * K.dkz8;
+  c.kx8 -= (K.NPK - 1) * K.dkx8;
+  c.ky8 -= (K.NPK - 1) * K.dky8;
+  c.zi8 += K.nz8;
 #pragma unroll
-  for (int x = 0; x < 3; ++x) c.f8[x] += K.nf8[x];
+  for (int x = 0; x < 3; ++x) c.fi8[x] += K.nf8[x];
   if (P.inner_is_g) {
     if (++c.g == P.ng) {
       c.g = 0;
       ++c.d;
       c.bco += K.L;
-      c.s8 -= (uint32_t)(P.ng - 1) * K.nz8;
+      c.si8 = 0;
     } else {
-      c.s8 += K.nz8;
+      c.si8 += K.nz8;
     }
   } else {
     if (++c.d == P.nd) {
       c.d = 0;
       ++c.g;
-      c.s8 += K.nz8;
+      c.si8 += K.nz8;
     }
     c.bco = c.d * K.L;
   }
@@ -227,7 +247,7 @@ __device__ __forceinline__ constexpr int memo(int m) {
 }
 
 template <int W, int V, int SX, bool POP, bool ADJ>
-__device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
+__device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds) {
   // steps of prefetch: 3 (2 with the population's extra registers)
   constexpr int RING = POP ? 2 : 3;
   const int nt = blockDim.x;
@@ -235,12 +255,11 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   const int l = tid % W;
   const int kk = tid / W;
   const int ni = P.ni, nj = P.nj, nk = P.nk, nd = P.nd, ng = P.ng;
-  const int PK = nk;
   const int nl = ni / V;                                // live lanes per row
   const bool live = l < nl;
   const int la = live ? l : 0;                          // address lane
   const int mb = SX > 0 ? la * V : ni - V - la * V;     // first zone (memory order) of the lane's block
-  const int k = P.kd > 0 ? kk : nk - 1 - kk;
+  const int k = P.kd > 0 ? kk : nk - 1 - kk;  // k-row in the first k-patch
 
   const uint32_t nz = (uint32_t)ni * nj * nk;
   const uint32_t nf[3] = {(uint32_t)nj * nk, (uint32_t)ni * nk, (uint32_t)ni * nj};
@@ -254,7 +273,11 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   K.jrow08 = 8 * j0 * ni;
   K.jstep8 = P.jd > 0 ? 8 : -8;
   K.jrowstep8 = K.jstep8 * ni;
-  K.rows = P.R * nj;
+  K.NPK = nk / PK;
+  K.dkz8 = (P.kd > 0 ? 8 : -8) * PK * nj * ni;
+  K.dkx8 = (P.kd > 0 ? 8 : -8) * PK * nj;
+  K.dky8 = (P.kd > 0 ? 8 : -8) * PK * ni;
+  K.rows = P.R * K.NPK * nj;
 
   // LDS: k-face exchange [2][nt][V], then one coefficient record per
   // direction: {2cos_x/dx [ni], 2cos_y/dy [nj], 2cos_z/dz [nk], w}
@@ -288,7 +311,7 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   const Rsrc Rpsi = rsrc(P.psi + p0 * nz, span_z);
   const Rsrc Rsig = rsrc(P.sig, (uint32_t)ng * K.nz8);
   const Rsrc Rvol = rsrc(P.vol, POP ? K.nz8 : 0u);
-  const uint32_t zk8 = 8u * (uint32_t)(k * nj * ni + mb);  // lane's zone in row j = 0
+  const uint32_t mb8 = 8u * (uint32_t)mb;
   Rsrc Rin[3], Rout[3];
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
@@ -296,6 +319,11 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
     Rin[x] = rsrc(P.in[x] ? P.in[x] + p0 * nf[x] : nullptr, P.in[x] ? span_f : 0u);  // vacuum: reads zeros
     Rout[x] = rsrc(P.out[x] + p0 * nf[x], span_f);
   }
+  // k-faces between k-patches go through the own k-plane; the first patch's
+  // incoming faces are there too when the subdomain has any (in[2] is the
+  // own plane or absent)
+  const bool has2 = P.in[2] != nullptr;
+  const Rsrc Rown2 = rsrc(P.own2 + p0 * nf[2], (uint32_t)P.R * K.nf8[2]);
 
   double rq[RING][V], rs[RING][V], rz[RING][V], ry[RING][V], rx[RING];
   // zone volumes (population) come from L2 one step ahead; that load is
@@ -303,27 +331,27 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
   double rv[V];
 
   RowCursor lc, cc;
-  cursor_init(lc, P, K, -kk, zk8, 8u * (uint32_t)(k * nj),
-              8u * (uint32_t)(k * ni + mb), 8u * (uint32_t)mb);
+  cursor_init(lc, P, K, -kk, 8u * (uint32_t)(k * nj * ni + mb), 8u * (uint32_t)(k * nj),
+              8u * (uint32_t)(k * ni + mb));
   cc = lc;
 
   auto refill = [&](int r) {
-    const uint32_t zoff = lc.z8 + lc.jrow8;
-    load_v<V, NT>(rq[r], Rrhs, zoff);
-    load_v<V, 0>(rs[r], Rsig, lc.s8 + lc.jrow8);
-    // faces: the k-face only for row group 0, the j-face only on an item's first row,
+    const uint32_t zrow = lc.kz8 + lc.jrow8;
+    load_v<V, NT>(rq[r], Rrhs, lc.zi8 + zrow);
+    load_v<V, 0>(rs[r], Rsig, lc.si8 + zrow);
+    // faces: the k-face only for row group 0, the j-face only on a patch's first row,
     // the x-face only for lane 0
-    load_v<V, 0>(rz[r], Rin[2], kk == 0 ? lc.f8[2] + lc.jrow8 : NOWHERE);
-    load_v<V, 0>(ry[r], Rin[1], lc.jj == 0 ? lc.f8[1] : NOWHERE);
+    load_v<V, 0>(rz[r], Rown2, (kk == 0 && (lc.kp > 0 || has2)) ? lc.fi8[2] + mb8 + lc.jrow8 : NOWHERE);
+    load_v<V, 0>(ry[r], Rin[1], lc.jj == 0 ? lc.fi8[1] + lc.ky8 : NOWHERE);
     {
       double t[1];
-      load_v<1, 0>(t, Rin[0], l == 0 ? lc.f8[0] + lc.j8 : NOWHERE);
+      load_v<1, 0>(t, Rin[0], l == 0 ? lc.fi8[0] + lc.kx8 + lc.j8 : NOWHERE);
       rx[r] = t[0];
     }
     cursor_step(lc, P, K);
   };
 
-  if (POP) load_v<V, 0>(rv, Rvol, zk8 + cc.jrow8);
+  if (POP) load_v<V, 0>(rv, Rvol, cc.kz8 + cc.jrow8);
 #pragma unroll
   for (int r = 0; r < RING; ++r) refill(r);
 
@@ -340,7 +368,8 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
       const int cur = t & 1, prev = cur ^ 1;
       const bool act = cc.pos >= 0 && cc.pos < K.rows;
       const double bj = co[cc.bco + ni + (int)(cc.j8 >> 3)];
-      const double ck = co[cc.bco + ni + nj + k];
+      const int kc = P.kd > 0 ? cc.kp * PK + kk : nk - 1 - cc.kp * PK - kk;  // this row group's k-row
+      const double ck = co[cc.bco + ni + nj + kc];
       const double bc = bj + ck;
       double a[V];
 #pragma unroll
@@ -419,17 +448,19 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, double *lds) {
         for (int e = 0; e < V; ++e) t += (wd * p[e]) * rv[e];
         pacc += st ? t : 0.0;
       }
-      store_v<V, NT>(Rpsi, st ? cc.z8 + cc.jrow8 : NOWHERE, p);
-      store_v<V, 0>(Rout[1], (st && cc.jj == nj - 1) ? cc.f8[1] : NOWHERE, fyo);
-      store_v<V, 0>(Rout[2], (st && kk == PK - 1) ? cc.f8[2] + cc.jrow8 : NOWHERE, fzo);
+      store_v<V, NT>(Rpsi, st ? cc.zi8 + cc.kz8 + cc.jrow8 : NOWHERE, p);
+      store_v<V, 0>(Rout[1], (st && cc.jj == nj - 1) ? cc.fi8[1] + cc.ky8 : NOWHERE, fyo);
+      const bool ktop = st && kk == PK - 1, klast = cc.kp == K.NPK - 1;
+      store_v<V, 0>(Rout[2], (ktop && klast) ? cc.fi8[2] + mb8 + cc.jrow8 : NOWHERE, fzo);
+      store_v<V, 0>(Rown2, (ktop && !klast) ? cc.fi8[2] + mb8 + cc.jrow8 : NOWHERE, fzo);
       {
         const double t[1] = {f};
-        store_v<1, 0>(Rout[0], (act && l == nl - 1) ? cc.f8[0] + cc.j8 : NOWHERE, t);
+        store_v<1, 0>(Rout[0], (act && l == nl - 1) ? cc.fi8[0] + cc.kx8 + cc.j8 : NOWHERE, t);
       }
 #pragma unroll
       for (int m = 0; m < V; ++m) fzb[(cur * nt + tid) * V + m] = fzo[memo<SX, V>(m)];
       cursor_step(cc, P, K);
-      if (POP) load_v<V, 0>(rv, Rvol, zk8 + cc.jrow8);
+      if (POP) load_v<V, 0>(rv, Rvol, cc.kz8 + cc.jrow8);
       refill(r);
       __syncthreads();
     }
@@ -483,11 +514,12 @@ __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__
   P.inner_is_g = sh.inner_is_g;
   P.w = S.w;
   P.vol = S.volume;
+  P.own2 = S.plane[2];
   P.pop = POP ? pop + blockIdx.x : nullptr;
   if ((S.id > 0) != ADJ)
-    rows_body<W, V, 1, POP, ADJ>(P, lds);
+    rows_body<W, V, 1, POP, ADJ>(P, sh.PK, lds);
   else
-    rows_body<W, V, -1, POP, ADJ>(P, lds);
+    rows_body<W, V, -1, POP, ADJ>(P, sh.PK, lds);
 }
 
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
@@ -495,7 +527,7 @@ int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *plan
                       double *pop, bool adjoint) {
   if (num_units <= 0) return KR_OK;
   const int W = shape.RG, V = shape.V;
-  const int nt = W * nk;
+  const int nt = W * shape.PK;
   const size_t lds = (2 * (size_t)nt * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
   switch (W * 8 + V) {
 #define X(w, v)                                                                                              \

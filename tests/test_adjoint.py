@@ -129,9 +129,11 @@ def test_scattering_zonal_matches_materials(ctx):
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
-@pytest.mark.parametrize("zones", [(8, 8, 8), (12, 6, 10)])
+@pytest.mark.parametrize("zones", [(8, 8, 8), (12, 6, 10), (128, 80, 128)])
 def test_sweep_transpose(ctx, layout, zones):
-    hp = product_for(ctx, **kwargs_for(layout, zset=(2, 2, 2), groups=8, zones=zones))
+    big = zones[0] > 64  # 64 x 40 x 64 subdomains: two k-patches in the row-scan sweep
+    hp = product_for(ctx, **kwargs_for(layout, zset=(2, 2, 2), groups=2 if big else 8, zones=zones,
+                                       quad=(4, 2) if big else (4, 4), legendre=1 if big else 2))
     k = hp.kernels()
     rng = np.random.default_rng(45)
     x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)

@@ -151,6 +151,26 @@ def test_sweep_solver_pipeline(ctx, layout):
     _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
 
 
+@pytest.mark.parametrize("layout", ZFAST)
+def test_sweep_kpatches(ctx, layout):
+    """A 64 x 40 x 64 subdomain: 32 lanes per row (2 zones each) x 64 k-rows
+    exceed one workgroup, so the row-scan sweep runs two k-patches per item
+    and hands the k-face between them through the own plane."""
+    kw = kwargs_for(layout, zones=(64, 40, 64), zset=(1, 1, 1), groups=2, quad=(4, 2), legendre=1)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(19)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=False)
+    # Kernel::sweepSubdomain form (own planes in and out) on the same subdomains
+    for s in range(hp.num_sdoms):
+        hp.kernels().sweep_subdomain(s)
+        of.sweep_subdomain(s)
+    _assert_chunks(hp, of, None, "psi", hp.num_sdoms, exact=False)
+
+
 @pytest.mark.parametrize("layout", ["DGZ", "ZGD"])
 def test_population(ctx, layout):
     kw = kwargs_for(layout, zset=(1, 2, 1), gset=2)
