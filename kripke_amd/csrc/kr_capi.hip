@@ -25,6 +25,7 @@ struct TuneChoice {
 struct kr_context {
   int device;
   hipStream_t stream;
+  size_t lds_max = 65536;  // LDS per workgroup (hipDeviceAttributeMaxSharedMemoryPerBlock)
   std::map<std::vector<int64_t>, TuneChoice> tuned;  // shape key -> choice
 };
 
@@ -130,6 +131,7 @@ LaunchCtx launch_ctx(kr_problem *p) {
   L.max_gmix = p->max_gmix;
   L.nd = p->h_sd.empty() ? 1 : p->h_sd[0].nd;
   L.dfast = (p->nesting == KR_GZD || p->nesting == KR_ZGD) ? 1 : 0;
+  L.lds_max = p->ctx->lds_max;
   return L;
 }
 
@@ -274,6 +276,9 @@ int kr_context_create(int32_t device, kr_context **out) {
   KR_HIP_CHECK(hipSetDevice(device));
   kr_context *c = new kr_context();
   c->device = device;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    c->lds_max = (size_t)lds;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -532,8 +537,12 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     for (const void *q : ptrs) aligned16 = aligned16 && ((uintptr_t)q % 16 == 0);
   }
   p->aligned16 = aligned16;
-  p->shape_ref = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 1, aligned16);
-  p->shape_pipe = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 0, aligned16);
+  {
+    SweepTune t;
+    t.lds_max = ctx->lds_max;
+    p->shape_ref = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 1, aligned16, t);
+    p->shape_pipe = choose_sweep_shape(p->nesting, nd, ng, ni, nj, nk, 0, aligned16, t);
+  }
   const int32_t pgs = (int32_t)(((int64_t)nd * ng) / p->shape_pipe.Q);
   std::vector<SweepUnit> uref, upipe;
   {
@@ -556,7 +565,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   }
 
   if (p->shape_ref.kind == 2 || p->shape_pipe.kind == 2)
-    return bad("sweep: directions per subdomain x (ni+nj+nk) exceed the 64 KiB LDS coefficient table");
+    return bad("sweep: directions per subdomain x (ni+nj+nk) exceed the LDS coefficient table");
   if ((rc = check_unit_spans(p, uref, p->shape_ref)) || (rc = check_unit_spans(p, upipe, p->shape_pipe))) {
     kr_problem_destroy(p);
     return rc;
@@ -898,6 +907,7 @@ static int install_choice(kr_problem *p, const TuneChoice &c) {
   SweepTune t;
   t.v_max = c.v_max;
   t.cap_div = c.cap_div;
+  t.lds_max = p->ctx->lds_max;
   const SweepShape sh = choose_sweep_shape(p->nesting, S.nd, S.ng, S.ni, S.nj, S.nk, 0, p->aligned16, t);
   if (sh.kind == 2) return fail(KR_ERR_UNSUPPORTED, "autotune: shape does not fit");
   std::vector<SweepUnit> u;

@@ -21,6 +21,7 @@ struct LaunchCtx {
   int num_pr, num_moments;
   int nd;                   // directions per subdomain (the same for every subdomain)
   int dfast;                // direction-fastest nesting (GZD, ZGD): psi [column][d], phi [column][nm]
+  size_t lds_max;           // LDS per workgroup on this device
   int64_t max_gz;           // max groups*zones over PR chunks
   int64_t max_gmix;         // max groups*mixed elements over PR chunks
 };
@@ -58,6 +59,8 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                       int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
                       double *pop, bool adjoint);
+// Lets `kernel` take `bytes` of dynamic LDS (needed above 64 KiB).
+int allow_lds(const void *kernel, size_t bytes);
 // Sum of n doubles in a fixed order (one workgroup): out[0] = sum(partial[0..n)).
 int launch_sum(hipStream_t st, const double *partial, int64_t n, double *out);
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).
@@ -69,6 +72,7 @@ int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 struct SweepTune {
   int v_max = 0;
   int cap_div = 1;
+  size_t lds_max = 65536;  // the device's LDS per workgroup (shapes within 64 KiB are preferred)
 };
 // aligned16: every psi/rhs/sigma_t/plane chunk starts on a 16-byte boundary.
 SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,

@@ -582,17 +582,20 @@ static int batch_of(int nd, int cap) {
 
 // LDS tile row stride for the direction-fastest kernels: odd, so that a
 // thread-per-row read of consecutive lanes hits distinct banks.  Tiles above
-// 64 KiB (very many directions or moments) fall back to the column kernels.
+// the device's LDS per workgroup fall back to the column kernels.
 static int dfast_stride(int nd, int M) { return std::max(nd, M) | 1; }
-static bool dfast_fits(int nd, int M) { return (size_t)DF_CB * dfast_stride(nd, M) * sizeof(double) <= 65536; }
+static bool dfast_fits(const LaunchCtx &L, int M) {
+  return (size_t)DF_CB * dfast_stride(L.nd, M) * sizeof(double) <= L.lds_max;
+}
 
 template <int M, bool ACC, bool ADJ>
 static void ltimes_launch(const LaunchCtx &L, dim3 grid, int db) {
-  if (L.dfast && dfast_fits(L.nd, M)) {
+  if (L.dfast && dfast_fits(L, M)) {
     const int ts = dfast_stride(L.nd, M);
     dim3 g((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
-    hipLaunchKernelGGL((k_ltimes_dfast<M, ACC, ADJ>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double), L.stream,
-                       L.sd, L.pr, L.pr_sdoms, ts);
+    const size_t lds = (size_t)DF_CB * ts * sizeof(double);
+    if (lds > 65536) allow_lds((const void *)k_ltimes_dfast<M, ACC, ADJ>, lds);
+    hipLaunchKernelGGL((k_ltimes_dfast<M, ACC, ADJ>), g, dim3(DF_CB), lds, L.stream, L.sd, L.pr, L.pr_sdoms, ts);
     return;
   }
   switch (db) {
@@ -606,11 +609,12 @@ static void ltimes_launch(const LaunchCtx &L, dim3 grid, int db) {
 
 template <int M, bool ACC, bool ADJ>
 static void lplus_launch(const LaunchCtx &L, dim3 grid, int db) {
-  if (L.dfast && dfast_fits(L.nd, M)) {
+  if (L.dfast && dfast_fits(L, M)) {
     const int ts = dfast_stride(L.nd, M);
     dim3 g((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
-    hipLaunchKernelGGL((k_lplustimes_dfast<M, ACC, ADJ>), g, dim3(DF_CB), (size_t)DF_CB * ts * sizeof(double),
-                       L.stream, L.sd, L.pr, L.pr_sdoms, ts);
+    const size_t lds = (size_t)DF_CB * ts * sizeof(double);
+    if (lds > 65536) allow_lds((const void *)k_lplustimes_dfast<M, ACC, ADJ>, lds);
+    hipLaunchKernelGGL((k_lplustimes_dfast<M, ACC, ADJ>), g, dim3(DF_CB), lds, L.stream, L.sd, L.pr, L.pr_sdoms, ts);
     return;
   }
   switch (db) {

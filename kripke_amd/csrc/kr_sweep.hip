@@ -381,6 +381,11 @@ __global__ __launch_bounds__(256) void k_sweep_coef(const DevSdom *__restrict__ 
   }
 }
 
+int allow_lds(const void *kernel, size_t bytes) {
+  KR_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return KR_OK;
+}
+
 int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count) {
   if (count <= 0) return KR_OK;
   hipLaunchKernelGGL(k_sweep_coef, dim3(4, count), dim3(256), 0, st, sd, first);
@@ -398,7 +403,7 @@ static int largest_divisor_le(int n, int cap) {
 // 16-byte aligned rows), W >= ni/V lanes per row group, one row group per
 // k-row.  Returns false when the subdomain does not fit one workgroup.
 static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, bool aligned16,
-                      int v_max) {
+                      int v_max, size_t lds_limit) {
   int want = ni % 2 == 0 ? 2 : 1;
   if (v_max > 0) want = std::min(want, v_max);
   const int64_t nz = (int64_t)ni * nj * nk;
@@ -418,7 +423,7 @@ static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_
     // it (4 steps ahead): nj >= PK + 4
     for (int PK = largest_divisor_le(nk, 1024 / W); PK >= 1; PK = largest_divisor_le(nk, PK - 1)) {
       const size_t lds = (2 * (size_t)W * PK * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
-      if ((PK == nk || nj >= PK + 4) && lds <= 65536) {
+      if ((PK == nk || nj >= PK + 4) && lds <= lds_limit) {
         sh.kind = 1;
         sh.RG = W;
         sh.V = V;
@@ -447,7 +452,10 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   // inner pair index = the faster of (d, g) in psi's memory order
   sh.inner_is_g = (nesting == KR_DGZ || nesting == KR_DZG || nesting == KR_ZDG) ? 1 : 0;
   const int inner = sh.inner_is_g ? ng : nd;
-  if (zfast && row_shape(sh, nd, ng, ni, nj, nk, aligned16, tune.v_max)) return sh;
+  // prefer shapes within 64 KiB of LDS (two workgroups per CU), then the device maximum
+  if (zfast && (row_shape(sh, nd, ng, ni, nj, nk, aligned16, tune.v_max, 65536) ||
+                (tune.lds_max > 65536 && row_shape(sh, nd, ng, ni, nj, nk, aligned16, tune.v_max, tune.lds_max))))
+    return sh;
   if (zfast) {
     sh.Q = 1;
   } else {
@@ -457,6 +465,8 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   }
   // LDS: j/k face exchange [4][threads] and the coefficient records [nd][ni+nj+nk+1]
   const size_t coef_lds = (size_t)nd * (ni + nj + nk + 1) * sizeof(double);
+  for (int pass = 0; pass < 2; ++pass) {
+  const size_t lds_limit = pass == 0 ? std::min<size_t>(65536, tune.lds_max) : tune.lds_max;
   for (int cap = 1024 / sh.Q / std::max(1, tune.cap_div); cap >= 1; cap /= 2) {
     // patch: as many (j,k) columns as fit, subject to the pipeline distances
     // (a patch's downstream neighbour prefetches a face COL_RING steps before
@@ -475,12 +485,13 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     }
     if (best_pj == 0) continue;
     const size_t lds = coef_lds + 4 * (size_t)sh.Q * best_pj * best_pk * sizeof(double);
-    if (lds > 65536) continue;
+    if (lds > lds_limit) continue;
     sh.PJ = best_pj;
     sh.PK = best_pk;
     sh.NPJ = nj / best_pj;
     sh.NPK = nk / best_pk;
     return sh;
+  }
   }
   sh.kind = 2;  // no shape fits: directions x zone extents too large for the LDS coefficient table
   return sh;
@@ -498,6 +509,12 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
                              adjoint);
   const int nt = shape.Q * shape.PJ * shape.PK;
   const size_t lds = (4 * (size_t)nt + (size_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1)) * sizeof(double);
+  if (lds > 65536) {
+    int rc;
+    if ((rc = allow_lds((const void *)k_sweep<false, true>, lds)) || (rc = allow_lds((const void *)k_sweep<true, false>, lds)) ||
+        (rc = allow_lds((const void *)k_sweep<false, false>, lds)))
+      return rc;
+  }
   if (adjoint)
     hipLaunchKernelGGL((k_sweep<false, true>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
   else if (pop)

@@ -532,6 +532,13 @@ int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *plan
   switch (W * 8 + V) {
 #define X(w, v)                                                                                              \
   case w * 8 + v:                                                                                            \
+    if (lds > 65536) {                                                                                       \
+      int rc;                                                                                                \
+      if ((rc = allow_lds((const void *)k_sweep_rows<w, v, false, true>, lds)) ||                            \
+          (rc = allow_lds((const void *)k_sweep_rows<w, v, true, false>, lds)) ||                            \
+          (rc = allow_lds((const void *)k_sweep_rows<w, v, false, false>, lds)))                             \
+        return rc;                                                                                           \
+    }                                                                                                        \
     if (adjoint)                                                                                             \
       hipLaunchKernelGGL((k_sweep_rows<w, v, false, true>), dim3(num_units), dim3(nt), lds, st, sd, planes,  \
                          units, shape, pop);                                                                 \

@@ -171,6 +171,21 @@ def test_sweep_kpatches(ctx, layout):
     _assert_chunks(hp, of, None, "psi", hp.num_sdoms, exact=False)
 
 
+@pytest.mark.parametrize("layout", ["DGZ", "DZG", "GZD"])
+def test_sweep_many_directions(ctx, layout):
+    """512 directions (64 per direction set) on a 32^3 subdomain: the sweep's
+    coefficient table (64 x 97 doubles) plus the face buffers need more than
+    64 KiB of LDS, which gfx950 allows per workgroup."""
+    kw = kwargs_for(layout, zones=(32, 32, 32), zset=(1, 1, 1), dset=8, groups=1, quad=(16, 32), legendre=1)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(21)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
+
+
 @pytest.mark.parametrize("layout", ["DGZ", "ZGD"])
 def test_population(ctx, layout):
     kw = kwargs_for(layout, zset=(1, 2, 1), gset=2)
