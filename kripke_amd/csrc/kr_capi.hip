@@ -75,7 +75,8 @@ struct kr_problem {
   const int64_t *m2l = nullptr;
   int64_t sigs_str[4] = {0, 0, 0, 0};
   int32_t *d_sigs_flags = nullptr;
-  int64_t max_gz = 0, max_gmix = 0;
+  double *d_sigs_t = nullptr;                 // dense scattering: sigs re-laid output-group fastest
+  int64_t max_gz = 0, max_gmix = 0, max_g = 0, max_z = 0;
   // timing
   int timing = 0;
   std::vector<TimedCall> pending;
@@ -125,12 +126,17 @@ LaunchCtx launch_ctx(kr_problem *p) {
   L.sigs_dims[2] = p->G;
   L.sigs_dims[3] = p->G;
   L.sigs_flags = p->d_sigs_flags;
+  L.sigs_t = p->d_sigs_t;
   L.num_pr = p->npr;
   L.num_moments = p->M;
+  L.num_r = p->nr;
   L.max_gz = p->max_gz;
   L.max_gmix = p->max_gmix;
+  L.max_g = p->max_g;
+  L.max_z = p->max_z;
   L.nd = p->h_sd.empty() ? 1 : p->h_sd[0].nd;
   L.dfast = (p->nesting == KR_GZD || p->nesting == KR_ZGD) ? 1 : 0;
+  L.gfast = (p->nesting == KR_DZG || p->nesting == KR_ZDG) ? 1 : 0;
   L.lds_max = p->ctx->lds_max;
   return L;
 }
@@ -488,6 +494,8 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   }
   std::vector<DevR> hr(p->nr);
   p->max_gz = (int64_t)ng * nz;
+  p->max_g = ng;
+  p->max_z = nz;
   p->max_gmix = 0;
   for (int c = 0; c < p->nr; ++c) {
     const kr_r_desc &R = p->rs[c];
@@ -509,9 +517,13 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     const bool n_first = (p->nesting == KR_DGZ || p->nesting == KR_DZG || p->nesting == KR_ZDG);
     if (n_first) { p->sigs_str[0] = L1 * G * G; p->sigs_str[1] = G * G; p->sigs_str[2] = G; p->sigs_str[3] = 1; }
     else { p->sigs_str[0] = G * G * L1; p->sigs_str[1] = 1; p->sigs_str[2] = G * L1; p->sigs_str[3] = L1; }
-    (void)nmat;
     std::vector<int32_t> flags(SIGS_FLAG_BLOCKS, 0);
     if ((rc = upload(ctx, &p->d_sigs_flags, flags))) { kr_problem_destroy(p); return rc; }
+    {
+      const int64_t nt = sigs_scratch_doubles(nmat, L1, G, p->M, p->nr, p->max_z);
+      const hipError_t e = hipMalloc((void **)&p->d_sigs_t, (size_t)nt * sizeof(double));
+      if (e != hipSuccess) { kr_problem_destroy(p); return fail(KR_ERR_NOMEM, std::string("hipMalloc sigs_t: ") + hipGetErrorString(e)); }
+    }
   }
 
   // --- sweep planes tables --------------------------------------------------
@@ -605,7 +617,7 @@ int kr_problem_destroy(kr_problem *p) {
   void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
                   p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
                   p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units,
-                  p->d_planes_adj,  (void *)p->d_ptrs};
+                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t};
   for (void *b : bufs)
     if (b) hipFree(b);
   delete p;

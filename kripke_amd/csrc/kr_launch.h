@@ -18,18 +18,23 @@ struct LaunchCtx {
   int64_t sigs_str[4];      // mat, n, g, gp strides of data/sigs
   int64_t sigs_dims[4];     // materials, legendre, groups, groups
   int32_t *sigs_flags;      // device, SIGS_FLAG_BLOCKS ints: nonzero = a coupling entry seen
-  int num_pr, num_moments;
+  double *sigs_t;           // device scratch of sigs_scratch_doubles(): sigs re-laid for the dense path, moment orders
+  int num_pr, num_moments, num_r;
   int nd;                   // directions per subdomain (the same for every subdomain)
   int dfast;                // direction-fastest nesting (GZD, ZGD): psi [column][d], phi [column][nm]
+  int gfast;                // group-fastest moment fields (DZG, ZDG): phi [..][z][g] / [z][nm][g]
   size_t lds_max;           // LDS per workgroup on this device
   int64_t max_gz;           // max groups*zones over PR chunks
   int64_t max_gmix;         // max groups*mixed elements over PR chunks
+  int64_t max_g, max_z;     // max groups and zones of one PR chunk
 };
 
 constexpr int SIGS_FLAG_BLOCKS = 64;
 
 int launch_ltimes(const LaunchCtx &L, int accumulate);
 int launch_lplustimes(const LaunchCtx &L, int accumulate);
+// doubles of LaunchCtx::sigs_t for the dense scattering path
+int64_t sigs_scratch_doubles(int64_t nmat, int64_t L1, int64_t G, int64_t M, int64_t nr, int64_t max_z);
 int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t *pr_r, int accumulate,
                       bool adjoint = false);
 // per-zone isotropic group-diagonal scattering from s(g,z) arrays (device array of one pointer per PR chunk)

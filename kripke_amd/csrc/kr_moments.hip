@@ -347,43 +347,297 @@ __global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ 
   }
 }
 
-// General sigs (group coupling, anisotropic moments): one thread per
-// (moment, group, zone) of the destination chunk, source groups in order.
-// ADJ: the transpose for the same (src, dst) chunk pair: phi_src(nm,gp,z)
-// (+)= sum over the destination chunk's groups g of sigs(n, g, gp) *
-// phi_out_dst(nm,g,z); the thread owns (nm, gp, z) of the source chunk.
-template <bool ADJ>
-__global__ __launch_bounds__(256) void k_scatter_dense(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
-                                                       const double *__restrict__ sigs, SigsStrides ss,
-                                                       const int64_t *__restrict__ m2l, int M, int src, int dst,
-                                                       int accumulate, const int32_t *__restrict__ flags) {
+// General sigs (group coupling, anisotropic moments).  The zone's matrix
+// sigs_z(n,g,gp) = sum over its mixed elements of frac * sigs(mat,n,g,gp) is
+// linear in the materials, so
+//     phi_out(nm,g,z) = sum_mat W(z,mat) sum_gp sigs(mat,n,g,gp) phi(nm,gp,z)
+// with W(z,mat) the zone's fraction of each material.  Per material this is
+// a dense (groups x groups) x (groups x zones) product, mapped so that a
+// wave's loads follow the nesting's fastest axis:
+//   zone fastest (DGZ, GDZ)    k_scatter_dense: lane = zone, one moment per
+//                              block, SG_OC output groups per lane;
+//   moment fastest (GZD, ZGD)  k_scatter_dense_mfast: lane = (zone, moment)
+//                              over the moments of one Legendre order, so
+//                              the coefficient row is still wave-uniform;
+//   group fastest (DZG, ZDG)   k_scatter_dense_gfast: lane = output group,
+//                              GF_ZT zones per lane, phi of a zone read
+//                              through the scalar cache.
+// In the first two the coefficient rows come through the scalar cache (one
+// load feeds SG_OC FMAs per loaded phi) and a wave skips every material none
+// of its zones contains.  The association differs from the reference's
+// per-zone matrix (rounding level).  k_sigs_relayout first writes the
+// coefficients as T[mat][n][h][o] with the output group o fastest: forward
+// o = g, h = gp; transpose o = gp, h = g.  T carries SG_PAD zeros at its end:
+// a partial output tile reads up to SG_OC - 1 entries past its row.
+constexpr int SG_OC = 16;
+constexpr int SG_PAD = 64;
+constexpr int GF_ZT = 16;  // zones per lane, group-fastest kernel
+constexpr int GF_HB = 8;   // source groups per scalar load, group-fastest kernel
+constexpr int MF_ZS = 252; // zones per supertile, moment-fastest kernel
+
+static int64_t tab_doubles(int64_t L1, int64_t M) { return (L1 + 1 + M + 1) / 2 + 1; }
+
+__global__ __launch_bounds__(256) void k_sigs_relayout(const double *__restrict__ sigs, SigsStrides ss, int64_t nmat,
+                                                       int64_t L1, int64_t G, double *__restrict__ T, int adj,
+                                                       const int32_t *__restrict__ flags) {
   if (!sigs_coupled(flags)) return;
-  const DevPR &Ps = prs[src];
-  const DevPR &Pd = prs[dst];
-  const DevR &R = rs[Pd.r];
-  const DevPR &Po = ADJ ? Ps : Pd;  // chunk owning the outputs
-  const DevPR &Pi = ADJ ? Pd : Ps;  // chunk read
-  const int64_t n = (int64_t)M * Po.ng * Po.nz;
-  const bool zfast = Po.mz <= Po.mg;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t per = (int64_t)Po.ng * Po.nz;
-    const int nm = (int)(e / per);
-    int32_t g, z;
-    gz_of(e % per, Po.ng, Po.nz, zfast, g, z);
-    const int64_t nl = m2l[nm];
-    const int64_t od = nm * Po.ma + g * Po.mg + z * Po.mz;
-    double *out = ADJ ? Po.phi : Po.phi_out;
-    const double *in = ADJ ? Pi.phi_out : Pi.phi;
-    double acc = accumulate ? out[od] : 0.0;
-    for (int32_t h = 0; h < Pi.ng; ++h) {
-      const double s = ADJ ? sigs_zone(sigs, ss, R, z, nl, Pd.glo + h, Ps.glo + g)
-                           : sigs_zone(sigs, ss, R, z, nl, Pd.glo + g, Ps.glo + h);
-      const double t = s * in[nm * Pi.ma + h * Pi.mg + z * Pi.mz];
-      acc = acc + t;
+  const int64_t n = nmat * L1 * G * G;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n + SG_PAD;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    if (e >= n) {
+      T[e] = 0.0;
+      continue;
     }
-    out[od] = acc;
+    const int64_t o = e % G, h = (e / G) % G, nl = (e / (G * G)) % L1, m = e / (G * G * L1);
+    const int64_t g = adj ? h : o, gp = adj ? o : h;
+    T[e] = sigs[m * ss.mat + nl * ss.n + g * ss.g + gp * ss.gp];
   }
 }
+
+// Moments grouped by Legendre order from moment_to_legendre: tab[n] is the
+// first slot of order n, tab[L1] the count, tab[L1 + 1 + slot] the moment.
+// Moments whose order lies outside [0, L1) take no part.
+__global__ void k_moment_orders(const int64_t *__restrict__ m2l, int M, int L1, int32_t *__restrict__ tab,
+                                const int32_t *__restrict__ flags) {
+  if (!sigs_coupled(flags) || threadIdx.x != 0 || blockIdx.x != 0) return;
+  int k = 0;
+  for (int n = 0; n < L1; ++n) {
+    tab[n] = k;
+    for (int nm = 0; nm < M; ++nm)
+      if (m2l[nm] == n) tab[L1 + 1 + k++] = nm;
+  }
+  tab[L1] = k;
+}
+
+// W[r][mat][z] (row stride wstride): the fraction of material mat in zone z
+// of zone set r, summed over the zone's mixed elements in their order.
+__global__ __launch_bounds__(256) void k_mat_weights(const DevR *__restrict__ rs, int64_t nmat, int64_t wstride,
+                                                     double *__restrict__ W, const int32_t *__restrict__ flags) {
+  if (!sigs_coupled(flags)) return;
+  const DevR &R = rs[blockIdx.y];
+  double *Wr = W + (int64_t)blockIdx.y * nmat * wstride;
+  for (int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; z < R.nz; z += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t m = 0; m < nmat; ++m) Wr[m * wstride + z] = 0.0;
+    const int64_t m0 = R.zone_to_mixelem[z];
+    const int32_t nmix = R.zone_to_num[z];
+    for (int32_t q = 0; q < nmix; ++q) {
+      const int64_t mat = R.mix_mat[m0 + q];
+      if (mat >= 0 && mat < nmat) Wr[mat * wstride + z] += R.mix_frac[m0 + q];
+    }
+  }
+}
+
+// acc[o] = sum_mat W(z,mat) sum_h T[mat][nl][gh_lo + h][go_lo + o0 + o] * in[h*mg]
+// for one lane (Wz = &W[r][0][z]); nl and the T offsets are wave-uniform.
+__device__ __forceinline__ void dense_lane(double (&acc)[SG_OC], const double *__restrict__ Wz, int64_t wstride,
+                                           bool live, const double *__restrict__ T, int64_t toff, int64_t nl,
+                                           int64_t nmat, int64_t L1, int64_t G, const gmem<double> *in, int64_t mg,
+                                           int nh) {
+#pragma unroll
+  for (int o = 0; o < SG_OC; ++o) acc[o] = 0.0;
+  for (int64_t mat = 0; mat < nmat; ++mat) {
+    const double wgt = live ? gp(Wz)[mat * wstride] : 0.0;
+    if (__ballot(wgt != 0.0) == 0) continue;  // no zone of this wave has the material
+    const cmem<double> *Tm = cp(T) + (mat * L1 + nl) * G * G + toff;
+    int h = 0;
+    for (; h + 4 <= nh; h += 4) {  // four loads in flight per step
+      double p[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) p[u] = in[(int64_t)(h + u) * mg];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double x = wgt * p[u];
+        const cmem<double> *row = Tm + (int64_t)(h + u) * G;
+#pragma unroll
+        for (int o = 0; o < SG_OC; ++o) acc[o] = __fma_rn(row[o], x, acc[o]);
+      }
+    }
+    for (; h < nh; ++h) {
+      const double x = wgt * in[(int64_t)h * mg];
+      const cmem<double> *row = Tm + (int64_t)h * G;
+#pragma unroll
+      for (int o = 0; o < SG_OC; ++o) acc[o] = __fma_rn(row[o], x, acc[o]);
+    }
+  }
+}
+
+__device__ __forceinline__ void dense_store(const double (&acc)[SG_OC], double *out, int64_t base, int o0, int ng,
+                                            int64_t mg, int accumulate) {
+#pragma unroll
+  for (int o = 0; o < SG_OC; ++o) {
+    if (o0 + o < ng) {
+      const int64_t od = base + (int64_t)(o0 + o) * mg;
+      out[od] = (accumulate ? out[od] : 0.0) + acc[o];
+    }
+  }
+}
+
+// Per (src, dst) chunk pair.  ADJ: the transpose for the same pair:
+// phi_src(nm,gp,z) (+)= sum over the destination chunk's groups g of
+// sigs_z(n,g,gp) * phi_out_dst(nm,g,z).
+#define KR_DENSE_PAIR                                                           \
+  const DevPR &Ps = prs[src];                                                   \
+  const DevPR &Pd = prs[dst];                                                   \
+  const double *Wr = W + (int64_t)Pd.r * nmat * wstride;                        \
+  const DevPR &Po = ADJ ? Ps : Pd; /* chunk owning the outputs */               \
+  const DevPR &Pi = ADJ ? Pd : Ps; /* chunk read */                             \
+  const int64_t go_lo = ADJ ? Ps.glo : Pd.glo, gh_lo = ADJ ? Pd.glo : Ps.glo;   \
+  double *out = ADJ ? Po.phi : Po.phi_out;                                      \
+  const double *inb = ADJ ? Pi.phi_out : Pi.phi;
+
+// Zone fastest: grid (zone blocks, moments x output tiles).
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_scatter_dense(const DevPR *__restrict__ prs, const double *__restrict__ W,
+                                                       int64_t wstride, const double *__restrict__ T,
+                                                       const int64_t *__restrict__ m2l,
+                                                       int M, int64_t nmat, int64_t L1, int64_t G, int src, int dst,
+                                                       int accumulate, const int32_t *__restrict__ flags) {
+  if (!sigs_coupled(flags)) return;
+  KR_DENSE_PAIR
+  const int noc = (Po.ng + SG_OC - 1) / SG_OC;
+  const int nm = blockIdx.y / noc;
+  const int o0 = (blockIdx.y - nm * noc) * SG_OC;
+  if (nm >= M || o0 >= Po.ng) return;
+  const int64_t nl = m2l[nm];
+  if (nl < 0 || nl >= L1) return;
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = z < Po.nz;
+  const int zc = live ? z : 0;
+  double acc[SG_OC];
+  dense_lane(acc, Wr + zc, wstride, live, T, gh_lo * G + go_lo + o0, nl, nmat, L1, G,
+             gp(inb) + nm * Pi.ma + zc * Pi.mz, Pi.mg, Pi.ng);
+  if (live) dense_store(acc, out, nm * Po.ma + (int64_t)z * Po.mz, o0, Po.ng, Po.mg, accumulate);
+}
+
+// Moment fastest: a wave covers zpw = 64 / mc zones x mc moments of one
+// order (mc = min(count, 64); orders with more than 64 moments take several
+// moment chunks), so its coefficient row is wave-uniform.  The zones are cut
+// into supertiles of MF_ZS; one block takes one (supertile, order, output
+// tile).  All (order, tile) blocks of a supertile read the same phi lines, so
+// the 1-D grid is laid out XCD-aware: blocks b, b + 8, b + 16, ... run on one
+// XCD, and consecutive blocks there take the nbt = orders x tiles blocks of
+// one supertile, which then share that XCD's L2.
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_scatter_dense_mfast(const DevPR *__restrict__ prs,
+                                                             const double *__restrict__ W, int64_t wstride,
+                                                             const double *__restrict__ T,
+                                                             const int32_t *__restrict__ tab, int64_t nmat,
+                                                             int64_t L1, int64_t G, int src, int dst, int accumulate,
+                                                             const int32_t *__restrict__ flags, int nocg) {
+  if (!sigs_coupled(flags)) return;
+  KR_DENSE_PAIR
+  const int64_t nbt = L1 * nocg;
+  const int64_t bi = blockIdx.x;
+  const int64_t xi = bi >> 3;
+  const int64_t st = (xi / nbt) * 8 + (bi & 7);
+  const int k = (int)(xi % nbt);
+  const int nl = k / nocg;
+  const int o0 = (k - nl * nocg) * SG_OC;
+  const int64_t zs0 = st * MF_ZS;
+  if (zs0 >= Po.nz || o0 >= Po.ng) return;
+  const int64_t zs1 = (zs0 + MF_ZS < Po.nz) ? zs0 + MF_ZS : Po.nz;
+  const int start = tab[nl], cnt = tab[nl + 1] - start;
+  if (cnt <= 0) return;
+  const int mc = cnt < 64 ? cnt : 64, nmc = (cnt + 63) / 64, zpw = 64 / mc;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int zl = lane / mc, ml = lane - zl * mc;
+  const int64_t nwork = ((zs1 - zs0 + zpw - 1) / zpw) * nmc;
+  for (int64_t w = wave; w < nwork; w += 4) {
+    const int64_t zg = w / nmc;
+    const int mk = (int)(w - zg * nmc) * 64 + ml;
+    const int64_t z = zs0 + zg * zpw + zl;
+    const bool live = zl < zpw && z < zs1 && mk < cnt;
+    const int nm = tab[L1 + 1 + start + (live ? mk : 0)];
+    const int64_t zc = live ? z : 0;
+    double acc[SG_OC];
+    dense_lane(acc, Wr + zc, wstride, live, T, gh_lo * G + go_lo + o0, nl, nmat, L1, G,
+               gp(inb) + nm * Pi.ma + zc * Pi.mz, Pi.mg, Pi.ng);
+    if (live) dense_store(acc, out, nm * Po.ma + z * Po.mz, o0, Po.ng, Po.mg, accumulate);
+  }
+}
+
+// Group fastest: grid (grid-stride over zone tiles of GF_ZT, moments x
+// 64-group output tiles); lane = output group.  Per material, a tile none of
+// whose zones holds it is skipped; the GF_HB phi values of one zone come in
+// one scalar load and each loaded coefficient feeds GF_ZT FMAs.  Per zone the
+// material's sum is formed first and then weighted: sum_mat W * (sum_h T phi).
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_scatter_dense_gfast(const DevPR *__restrict__ prs,
+                                                             const double *__restrict__ W, int64_t wstride,
+                                                             const double *__restrict__ T,
+                                                             const int64_t *__restrict__ m2l, int M, int64_t nmat,
+                                                             int64_t L1, int64_t G, int src, int dst, int accumulate,
+                                                             const int32_t *__restrict__ flags) {
+  if (!sigs_coupled(flags)) return;
+  KR_DENSE_PAIR
+  const int noc = (Po.ng + 63) / 64;
+  const int nm = blockIdx.y / noc;
+  const int o0 = (blockIdx.y - nm * noc) * 64;
+  if (nm >= M || o0 >= Po.ng) return;
+  const int64_t nl = m2l[nm];
+  if (nl < 0 || nl >= L1) return;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int o = o0 + lane;
+  const bool olive = o < Po.ng;
+  const int64_t ntile = ((int64_t)Po.nz + GF_ZT - 1) / GF_ZT;
+  const int nh = Pi.ng;
+  for (int64_t t = (int64_t)blockIdx.x * 4 + wave; t < ntile; t += (int64_t)gridDim.x * 4) {
+    const int64_t z0 = t * GF_ZT;
+    const int nzt = (int)((Po.nz - z0) < GF_ZT ? (Po.nz - z0) : GF_ZT);
+    double acc[GF_ZT];
+#pragma unroll
+    for (int zt = 0; zt < GF_ZT; ++zt) acc[zt] = 0.0;
+    for (int64_t mat = 0; mat < nmat; ++mat) {
+      double wz[GF_ZT];
+      bool any = false;
+      const cmem<double> *wrow = cp(Wr) + mat * wstride + z0;  // wstride >= zones + GF_ZT: a tile's row stays inside
+#pragma unroll
+      for (int zt = 0; zt < GF_ZT; ++zt) {
+        wz[zt] = zt < nzt ? wrow[zt] : 0.0;
+        any |= wz[zt] != 0.0;
+      }
+      if (!any) continue;  // zone-uniform: no zone of the tile has the material
+      double inner[GF_ZT];
+#pragma unroll
+      for (int zt = 0; zt < GF_ZT; ++zt) inner[zt] = 0.0;
+      const gmem<double> *Tc = gp(T) + (mat * L1 + nl) * G * G + gh_lo * G + go_lo + (olive ? o : o0);
+      const double *in0 = inb + nm * Pi.ma + z0 * Pi.mz;
+      int h = 0;
+      for (; h + GF_HB <= nh; h += GF_HB) {
+        double tc[GF_HB];
+#pragma unroll
+        for (int u = 0; u < GF_HB; ++u) tc[u] = Tc[(int64_t)(h + u) * G];
+#pragma unroll
+        for (int zt = 0; zt < GF_ZT; ++zt) {
+          const int zr = zt < nzt ? zt : 0;
+          const cmem<double> *x = cp(in0) + (int64_t)zr * Pi.mz + h;
+#pragma unroll
+          for (int u = 0; u < GF_HB; ++u) inner[zt] = __fma_rn(tc[u], x[u], inner[zt]);
+        }
+      }
+      for (; h < nh; ++h) {
+        const double tc = Tc[(int64_t)h * G];
+#pragma unroll
+        for (int zt = 0; zt < GF_ZT; ++zt) {
+          const int zr = zt < nzt ? zt : 0;
+          inner[zt] = __fma_rn(tc, cp(in0)[(int64_t)zr * Pi.mz + h], inner[zt]);
+        }
+      }
+#pragma unroll
+      for (int zt = 0; zt < GF_ZT; ++zt) acc[zt] = __fma_rn(wz[zt], inner[zt], acc[zt]);
+    }
+    if (olive) {
+#pragma unroll
+      for (int zt = 0; zt < GF_ZT; ++zt) {
+        if (zt < nzt) {
+          const int64_t od = nm * Po.ma + (int64_t)o * Po.mg + (z0 + zt) * Po.mz;
+          out[od] = (accumulate ? out[od] : 0.0) + acc[zt];
+        }
+      }
+    }
+  }
+}
+#undef KR_DENSE_PAIR
 
 // Scattering with a per-zone, isotropic, group-diagonal cross section s(g,z)
 // given directly (one array per PR chunk in sigt_zonal layout) instead of
@@ -673,10 +927,27 @@ int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t 
     hipLaunchKernelGGL(k_scatter_diag<false>, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments,
                        accumulate, (const int32_t *)L.sigs_flags);
   KR_HIP_CHECK(hipGetLastError());
-  // Dense: for each source chunk (work-list order) and each destination
-  // chunk on the same zone set (Scattering.cpp:137-145).  The transpose
-  // writes the source chunks: the first pair that writes a chunk overwrites
-  // (unless accumulating), later pairs add.
+  // Dense: coefficient re-layout, then for each source chunk (work-list
+  // order) and each destination chunk on the same zone set
+  // (Scattering.cpp:137-145).  The transpose writes the source chunks: the
+  // first pair that writes a chunk overwrites (unless accumulating), later
+  // pairs add.
+  const int64_t nmat = L.sigs_dims[0], L1 = L.sigs_dims[1], G = L.sigs_dims[2];
+  const int M = L.num_moments;
+  const int32_t *flags = (const int32_t *)L.sigs_flags;
+  hipLaunchKernelGGL(k_sigs_relayout, dim3(grid_for(nmat * L1 * G * G + SG_PAD)), dim3(256), 0, L.stream, L.sigs, ss,
+                     nmat, L1, G, L.sigs_t, adjoint ? 1 : 0, flags);
+  KR_HIP_CHECK(hipGetLastError());
+  int32_t *tab = (int32_t *)(L.sigs_t + nmat * L1 * G * G + SG_PAD);
+  const int64_t wstride = L.max_z + GF_ZT;
+  double *W = L.sigs_t + nmat * L1 * G * G + SG_PAD + tab_doubles(L1, M);
+  hipLaunchKernelGGL(k_mat_weights, dim3(grid_for(L.max_z), L.num_r), dim3(256), 0, L.stream, L.r, nmat, wstride, W,
+                     flags);
+  KR_HIP_CHECK(hipGetLastError());
+  if (L.dfast) {
+    hipLaunchKernelGGL(k_moment_orders, dim3(1), dim3(64), 0, L.stream, L.m2l, M, (int)L1, tab, flags);
+    KR_HIP_CHECK(hipGetLastError());
+  }
   std::vector<int> first(L.num_pr, 1);
   for (int s = 0; s < L.num_pr; ++s)
     for (int d = 0; d < L.num_pr; ++d) {
@@ -684,16 +955,49 @@ int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t 
       const int w = adjoint ? s : d;
       const int acc = first[w] ? accumulate : 1;
       first[w] = 0;
-      if (adjoint)
-        hipLaunchKernelGGL(k_scatter_dense<true>, dim3(grid_for(L.max_gz * L.num_moments)), dim3(256), 0, L.stream,
-                           L.pr, L.r, L.sigs, ss, L.m2l, L.num_moments, s, d, acc, (const int32_t *)L.sigs_flags);
-      else
-        hipLaunchKernelGGL(k_scatter_dense<false>, dim3(grid_for(L.max_gz * L.num_moments)), dim3(256), 0, L.stream,
-                           L.pr, L.r, L.sigs, ss, L.m2l, L.num_moments, s, d, acc, (const int32_t *)L.sigs_flags);
+      if (L.dfast) {
+        const int nocg = (int)((L.max_g + SG_OC - 1) / SG_OC);
+        const int64_t nst = (L.max_z + MF_ZS - 1) / MF_ZS;
+        const int64_t nblk = ((nst + 7) / 8) * 8 * L1 * nocg;
+        if (nblk > 0x7fffffffLL) {
+          set_error("scattering: too many zones for the dense moment-fastest grid");
+          return KR_ERR_UNSUPPORTED;
+        }
+        if (adjoint)
+          hipLaunchKernelGGL(k_scatter_dense_mfast<true>, dim3((unsigned)nblk), dim3(256), 0, L.stream, L.pr, W,
+                             wstride, L.sigs_t, tab, nmat, L1, G, s, d, acc, flags, nocg);
+        else
+          hipLaunchKernelGGL(k_scatter_dense_mfast<false>, dim3((unsigned)nblk), dim3(256), 0, L.stream, L.pr, W,
+                             wstride, L.sigs_t, tab, nmat, L1, G, s, d, acc, flags, nocg);
+      } else if (L.gfast) {
+        const int64_t tiles = (L.max_z + GF_ZT - 1) / GF_ZT;
+        dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1 << 20)),
+                  (unsigned)(M * ((L.max_g + 63) / 64)));
+        if (adjoint)
+          hipLaunchKernelGGL(k_scatter_dense_gfast<true>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l,
+                             M, nmat, L1, G, s, d, acc, flags);
+        else
+          hipLaunchKernelGGL(k_scatter_dense_gfast<false>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l,
+                             M, nmat, L1, G, s, d, acc, flags);
+      } else {
+        dim3 grid((unsigned)((L.max_z + 255) / 256), (unsigned)(M * ((L.max_g + SG_OC - 1) / SG_OC)));
+        if (adjoint)
+          hipLaunchKernelGGL(k_scatter_dense<true>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l, M,
+                             nmat, L1, G, s, d, acc, flags);
+        else
+          hipLaunchKernelGGL(k_scatter_dense<false>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l, M,
+                             nmat, L1, G, s, d, acc, flags);
+      }
       KR_HIP_CHECK(hipGetLastError());
     }
   (void)pr_gset;
   return KR_OK;
+}
+
+// Dense-path scratch: T (+ SG_PAD), the moment-order table (int32), then the
+// weight table W[r][mat][z] with row stride max_z + GF_ZT.
+int64_t sigs_scratch_doubles(int64_t nmat, int64_t L1, int64_t G, int64_t M, int64_t nr, int64_t max_z) {
+  return nmat * L1 * G * G + SG_PAD + tab_doubles(L1, M) + nr * nmat * (max_z + GF_ZT);
 }
 
 int launch_scattering_zonal(const LaunchCtx &L, const double *const *sz_dev, int64_t s_g, int64_t s_z, int accumulate,

@@ -76,7 +76,7 @@ def test_lplustimes_transpose(ctx, layout):
 
 
 @pytest.mark.parametrize("dense", [False, True])
-@pytest.mark.parametrize("layout", ["DGZ", "GZD", "ZDG"])
+@pytest.mark.parametrize("layout", ["DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD"])
 def test_scattering_transpose(ctx, layout, dense):
     hp = product_for(ctx, **kwargs_for(layout, zset=(1, 1, 2), gset=2, groups=4, legendre=1))
     k = hp.kernels()
