@@ -141,6 +141,18 @@ int kr_sweep_subdomain(kr_problem *prob, int32_t sdom_id);
  * vacuum faces read as zero, outgoing faces written straight into the
  * downwind subdomain's plane chunk.  Upwind subdomains must be listed. */
 int kr_sweep_solver(kr_problem *prob, const int32_t *sdom_list, int32_t num);
+/* SweepSolver with block_jacobi = true (--pmethod bj): SweepSolver.cpp:23-87
+ * with BlockJacobiComm (src/Kripke/ParallelComm/BlockJacobiComm.cpp:23-116)
+ * on one rank.  The reference's communicator allocates fresh old_*_plane
+ * fields (BlockJacobiComm.cpp:31-33; new double[], Field.h:68, so their
+ * contents are indeterminate there) and copies them into the live planes
+ * (the reversed kCopy, Kernel.h:63-80) and into each downwind subdomain
+ * (ParallelComm.cpp:137-162) before any sweep.  Here the old planes are zero,
+ * so every listed subdomain is swept with zero incoming faces, independently
+ * of the others (one launch), and leaves its outgoing faces in its own plane
+ * chunks; plane chunks of unlisted subdomains end zero.  Ids must be
+ * distinct. */
+int kr_sweep_solver_bj(kr_problem *prob, const int32_t *sdom_list, int32_t num);
 /* SweepSolver followed by Kernel::population, the two calls the reference
  * makes back to back (src/Kripke/SteadyStateSolver.cpp:92-99).  The list
  * must name every subdomain once.  The population is summed by the sweep
@@ -243,6 +255,7 @@ typedef struct kr_input_vars {
   uint64_t seed;
   double sigt[3], sigs[3];           /* used by geometry 0 */
   double source_strength;            /* Source.cpp:98 uses 1.0 */
+  int32_t parallel_method;           /* InputVariables.h:42: 0 = sweep, 1 = Block Jacobi (--pmethod bj) */
 } kr_input_vars;
 
 typedef struct kr_host_problem kr_host_problem;
@@ -265,6 +278,8 @@ int kr_host_field_device(kr_host_problem *hp, const char *name, int32_t chunk, v
  *         points, host-side SweepComm queue over kr_sweep_subdomain).
  * mode 1: the on-device pipeline (fills folded into the kernels,
  *         kr_sweep_solver).  Both give the same phi and psi.
+ * parallel_method 1 sweeps with Block Jacobi in both modes
+ * (SteadyStateSolver.cpp:82-92 passes block_jacobi to SweepSolver).
  * kr_host_init_psi zeroes psi (SteadyStateSolver.cpp:39). */
 int kr_host_init_psi(kr_host_problem *hp);
 int kr_host_iterate(kr_host_problem *hp, int32_t mode, double *population);

@@ -40,15 +40,17 @@ class InputVars(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("sigt", ctypes.c_double * 3), ("sigs", ctypes.c_double * 3),
         ("source_strength", ctypes.c_double),
+        ("parallel_method", ctypes.c_int32),
     ]
 
 
 def input_vars(zones=(16, 16, 16), groups=32, quad=(8, 12), directions=None, legendre=4, layout="DGZ",
                dset=8, gset=1, zset=(1, 1, 1), geometry=1, seed=20261202, subsamples=4,
-               sigt=(0.1, 0.0001, 0.1), sigs=(0.05, 0.00005, 0.05), source_strength=1.0):
+               sigt=(0.1, 0.0001, 0.1), sigs=(0.05, 0.00005, 0.05), source_strength=1.0, pmethod="sweep"):
     """InputVariables with the reference's meaning (src/Kripke/InputVariables.cpp:17-46).
     quad=(polar, azimuthal) is the Gauss-Legendre product set; quad=None the
-    dummy S2 set with `directions` points."""
+    dummy S2 set with `directions` points; pmethod is --pmethod, "sweep" or
+    "bj" (src/kripke.cpp:329-340)."""
     v = InputVars()
     v.nx, v.ny, v.nz = zones
     if quad is None:
@@ -71,6 +73,9 @@ def input_vars(zones=(16, 16, 16), groups=32, quad=(8, 12), directions=None, leg
         v.sigt[m] = sigt[m]
         v.sigs[m] = sigs[m]
     v.source_strength = source_strength
+    if pmethod not in ("sweep", "bj"):
+        raise ValueError("pmethod must be 'sweep' or 'bj'")
+    v.parallel_method = 1 if pmethod == "bj" else 0
     return v
 
 
@@ -98,6 +103,7 @@ SIGNATURES = [
     ("kr_source", ctypes.c_int, [P, ctypes.c_double]),
     ("kr_sweep_subdomain", ctypes.c_int, [P, ctypes.c_int32]),
     ("kr_sweep_solver", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    ("kr_sweep_solver_bj", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     ("kr_sweep_solver_population", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, DP]),
     ("kr_population", ctypes.c_int, [P, DP]),
     ("kr_ltimes_adjoint", ctypes.c_int, [P]),
@@ -212,9 +218,12 @@ class KernelProblem:
     def sweep_subdomain(self, sdom):
         check(lib().kr_sweep_subdomain(self.h, sdom), "kr_sweep_subdomain")
 
-    def sweep_solver(self, sdoms):
+    def sweep_solver(self, sdoms, block_jacobi=False):
         arr = (ctypes.c_int32 * len(sdoms))(*sdoms)
-        check(lib().kr_sweep_solver(self.h, arr, len(sdoms)), "kr_sweep_solver")
+        if block_jacobi:
+            check(lib().kr_sweep_solver_bj(self.h, arr, len(sdoms)), "kr_sweep_solver_bj")
+        else:
+            check(lib().kr_sweep_solver(self.h, arr, len(sdoms)), "kr_sweep_solver")
 
     def sweep_solver_population(self, sdoms):
         arr = (ctypes.c_int32 * len(sdoms))(*sdoms)

@@ -217,6 +217,7 @@ void check_inputs(const InputVariables &in) {  // InputVariables.cpp:52-109
     bad("Number of zones must evenly divide into the number of subdomains");  // Space.cpp:49,65,80
   if (in.num_material_subsamples < 1) bad("num_material_subsamples must be >= 1");
   if (in.geometry != 0 && in.geometry != 1) bad("geometry must be 0 (Kobayashi) or 1 (seeded boxes)");
+  if (in.parallel_method != 0 && in.parallel_method != 1) bad("parallel_method must be 0 (sweep) or 1 (bj)");
 }
 
 }  // namespace

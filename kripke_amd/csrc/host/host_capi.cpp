@@ -16,6 +16,7 @@ struct kr_host_problem {
   DataStore ds;
   kr_context *ctx = nullptr;
   int32_t nsdom = 0, npr = 0, nr = 0;
+  bool block_jacobi = false;  // InputVariables::parallel_method == PMETHOD_BJ
 };
 
 namespace {
@@ -72,6 +73,7 @@ int kr_host_create(kr_context *ctx, const kr_input_vars *in, kr_host_problem **o
   }
   kr_host_problem *hp = new kr_host_problem();
   hp->ctx = ctx;
+  hp->block_jacobi = in->parallel_method == 1;
   int rc = guarded([&] {
     generateProblem(hp->ds, *in, ctx);
     auto &ps = hp->ds.getVariable<PartitionSpace>("pspace");
@@ -169,7 +171,7 @@ int kr_host_iterate(kr_host_problem *hp, int32_t mode, double *population) {
     return KR_ERR_INVALID;
   }
   return guarded([&] {
-    const double p = SteadyStateIteration(hp->ds, mode);
+    const double p = SteadyStateIteration(hp->ds, mode, hp->block_jacobi);
     if (population) *population = p;
     return KR_OK;
   });

@@ -67,10 +67,11 @@ void LPlusTimesOverwrite(Core::DataStore &ds);
 void scatteringOverwrite(Core::DataStore &ds);
 }  // namespace Kernel
 
-// src/Kripke/SweepSolver.cpp:23-87.  pipeline=true uses kr_sweep_solver.
-void SweepSolver(Core::DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline);
+// src/Kripke/SweepSolver.cpp:23-87.  pipeline=true uses kr_sweep_solver
+// (kr_sweep_solver_bj when block_jacobi).
+void SweepSolver(Core::DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline, bool block_jacobi = false);
 // One iteration of src/Kripke/SteadyStateSolver.cpp:44-104.
-double SteadyStateIteration(Core::DataStore &ds, int mode);
+double SteadyStateIteration(Core::DataStore &ds, int mode, bool block_jacobi = false);
 
 // Thrown on any C-ABI failure inside the host mirror (KRIPKE_ABORT analogue).
 struct KripkeError : public std::runtime_error {

@@ -41,12 +41,16 @@ double Kernel::population(DataStore &ds) {
   return part;  // Comm::allReduceSumDouble is the identity on one rank
 }
 
-// SweepSolver with the on-rank SweepComm (SweepSolver.cpp:23-87,
-// ParallelComm.cpp:61-263).  pipeline=false replays the reference queue on
-// the host; pipeline=true hands the whole list to the on-device pipeline.
-void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline) {
+// SweepSolver (SweepSolver.cpp:23-87) with the on-rank SweepComm
+// (ParallelComm.cpp:61-263) or, block_jacobi, BlockJacobiComm
+// (BlockJacobiComm.cpp:23-116).  pipeline=false replays the reference queue
+// on the host; pipeline=true hands the whole list to one device call.
+void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline, bool block_jacobi) {
   if (pipeline) {
-    check(kr_sweep_solver(kp(ds), sdom_list.data(), (int32_t)sdom_list.size()), "SweepSolver");
+    if (block_jacobi)
+      check(kr_sweep_solver_bj(kp(ds), sdom_list.data(), (int32_t)sdom_list.size()), "SweepSolver");
+    else
+      check(kr_sweep_solver(kp(ds), sdom_list.data(), (int32_t)sdom_list.size()), "SweepSolver");
     return;
   }
   auto &ipl = ds.getVariable<Field<double>>("i_plane");
@@ -57,6 +61,18 @@ void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bo
   Kernel::kConst(jpl, 0.0);
   Kernel::kConst(kpl, 0.0);
   auto &up = ds.getVariable<Field<int32_t>>("upwind");
+  if (block_jacobi) {
+    // The communicator's fresh old planes are zero; the reversed kCopy in
+    // addSubdomain and the on-rank sends of the first workRemaining() copy
+    // them into every plane chunk, which are zero already.  Then every
+    // subdomain is ready: FIFO, vacuum faces cleared, no sends on completion.
+    for (SdomId s : sdom_list) {
+      for (int dim = 0; dim < 3; ++dim)
+        if (up.host(s)[dim] == -1) Kernel::kConst(*planes[dim], s, 0.0);
+      Kernel::sweepSubdomain(ds, s);
+    }
+    return;
+  }
   auto &down = ds.getVariable<Field<int32_t>>("downwind");
   std::vector<SdomId> q_ids;
   std::vector<int> q_dep;
@@ -99,7 +115,7 @@ void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bo
 // SteadyStateSolver.cpp:44-104.  mode 0: the reference call sequence;
 // mode 1: fills folded into the kernels, the on-device sweep pipeline, and
 // the population summed while the sweep writes psi.
-double Kripke::SteadyStateIteration(DataStore &ds, int mode) {
+double Kripke::SteadyStateIteration(DataStore &ds, int mode, bool block_jacobi) {
   auto &pspace = ds.getVariable<PartitionSpace>("pspace");
   const int n = (int)pspace.getNumSubdomains(SPACE_PQR);
   std::vector<SdomId> all(n);
@@ -112,12 +128,16 @@ double Kripke::SteadyStateIteration(DataStore &ds, int mode) {
     Kernel::source(ds);
     Kernel::kConst(ds.getVariable<Field<double>>("rhs"), 0.0);
     Kernel::LPlusTimes(ds);
-    SweepSolver(ds, all, false);
+    SweepSolver(ds, all, false, block_jacobi);
   } else {
     Kernel::LTimesOverwrite(ds);
     Kernel::scatteringOverwrite(ds);
     Kernel::source(ds);
     Kernel::LPlusTimesOverwrite(ds);
+    if (block_jacobi) {
+      SweepSolver(ds, all, true, true);
+      return Kernel::population(ds);
+    }
     double part = 0.0;  // SweepSolver + population in one pass over psi
     check(kr_sweep_solver_population(kp(ds), all.data(), (int32_t)all.size(), &part), "SweepSolver");
     return part;

@@ -63,6 +63,7 @@ struct kr_problem {
   double *d_pop_units = nullptr;               // fused population: one partial per sweep unit
   int64_t pop_units_cap = 0;
   SweepPlanes *d_planes_adj = nullptr;         // adjoint sweep: faces from the forward-downwind side
+  SweepPlanes *d_planes_bj = nullptr;          // Block Jacobi: no incoming faces, outgoing faces kept in place
   static constexpr int PTR_SLOTS = 4;          // zonal sigma_s, sigma_t gradient, sigma_s gradient, spare
   const double **d_ptrs = nullptr;             // PTR_SLOTS per-PR-chunk pointer arrays passed to kernels
   std::vector<const double *> ptrs_last[PTR_SLOTS];  // their current contents
@@ -527,7 +528,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   }
 
   // --- sweep planes tables --------------------------------------------------
-  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom), padj(p->nsdom);
+  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom), padj(p->nsdom), pbj(p->nsdom);
   for (int s = 0; s < p->nsdom; ++s) {
     const DevSdom &H = p->h_sd[s];
     for (int dim = 0; dim < 3; ++dim) {
@@ -539,6 +540,8 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
       // forward-downwind neighbour and leave toward the forward-upwind one
       padj[s].in[dim] = H.down[dim] >= 0 ? H.plane[dim] : nullptr;
       padj[s].out[dim] = H.up[dim] >= 0 ? p->h_sd[H.up[dim]].plane[dim] : H.plane[dim];
+      pbj[s].in[dim] = nullptr;
+      pbj[s].out[dim] = H.plane[dim];
     }
   }
 
@@ -588,6 +591,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   if ((rc = upload(ctx, &p->d_sd, hsd)) || (rc = upload(ctx, &p->d_pr, p->h_pr)) || (rc = upload(ctx, &p->d_r, hr)) ||
       (rc = upload(ctx, &p->d_pr_sdoms, pr_sdoms)) || (rc = upload(ctx, &p->d_planes_ref, pref)) ||
       (rc = upload(ctx, &p->d_planes_pipe, ppipe)) || (rc = upload(ctx, &p->d_planes_adj, padj)) ||
+      (rc = upload(ctx, &p->d_planes_bj, pbj)) ||
       (rc = upload(ctx, &p->d_units_ref, uref)) ||
       (rc = upload(ctx, &p->d_units_pipe, upipe))) {
     kr_problem_destroy(p);
@@ -617,7 +621,7 @@ int kr_problem_destroy(kr_problem *p) {
   void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
                   p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
                   p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units,
-                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t};
+                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t, p->d_planes_bj};
   for (void *b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -733,6 +737,51 @@ int kr_sweep_solver(kr_problem *p, const int32_t *list, int32_t num) {
   if (!p || (!list && num > 0) || num < 0) return fail(KR_ERR_INVALID, "kr_sweep_solver: bad argument");
   CallTimer t(p, "SweepSolver");
   return sweep_solver_impl(p, list, num, nullptr, nullptr);
+}
+
+// SweepSolver with BlockJacobiComm on one rank (SweepSolver.cpp:23-87,
+// BlockJacobiComm.cpp:23-116): every plane chunk starts from zero (the
+// SweepSolver fill plus the communicator's fresh old planes, copied in by the
+// reversed kCopy and the on-rank sends), so each listed subdomain is swept
+// with zero incoming faces, independently of the others: one launch.
+int kr_sweep_solver_bj(kr_problem *p, const int32_t *list, int32_t num) {
+  if (!p || (!list && num > 0) || num < 0) return fail(KR_ERR_INVALID, "kr_sweep_solver_bj: bad argument");
+  std::vector<char> listed(p->nsdom, 0);
+  for (int32_t i = 0; i < num; ++i) {
+    if (list[i] < 0 || list[i] >= p->nsdom || listed[list[i]]++)
+      return fail(KR_ERR_INVALID, "kr_sweep_solver_bj: subdomain ids must be valid and distinct");
+  }
+  CallTimer t(p, "SweepSolver");
+  // planes of unlisted subdomains end as the zero old planes; listed ones
+  // are overwritten by their own outgoing faces
+  for (int s = 0; s < p->nsdom; ++s) {
+    if (listed[s]) continue;
+    const kr_sdom_desc &S = p->sdoms[s];
+    const int64_t dg = (int64_t)S.num_directions * S.num_groups;
+    const int64_t n[3] = {dg * S.nj * S.nk, dg * S.ni * S.nk, dg * S.ni * S.nj};
+    for (int dim = 0; dim < 3; ++dim) {
+      int rc = launch_fill(p->ctx->stream, p->h_sd[s].plane[dim], n[dim], 0.0);
+      if (rc) return rc;
+    }
+  }
+  if (num == 0) return KR_OK;
+  int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, 0, p->nsdom);
+  if (rc) return rc;
+  const int32_t pgs = (int32_t)(((int64_t)p->h_sd[0].nd * p->h_sd[0].ng) / p->shape_pipe.Q);
+  const int32_t R = pick_R((int64_t)pgs * num, p->shape_pipe);
+  std::vector<SweepUnit> u;
+  for (int32_t i = 0; i < num; ++i) append_units(u, list[i], pgs, R);
+  if ((rc = check_unit_spans(p, u, p->shape_pipe))) return rc;
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
+  if ((int32_t)u.size() > p->units_tmp_cap) {
+    if (p->d_units_tmp) KR_HIP_CHECK(hipFree(p->d_units_tmp));
+    p->d_units_tmp = nullptr;
+    KR_HIP_CHECK(hipMalloc((void **)&p->d_units_tmp, u.size() * sizeof(SweepUnit)));
+    p->units_tmp_cap = (int32_t)u.size();
+  }
+  KR_HIP_CHECK(hipMemcpy(p->d_units_tmp, u.data(), u.size() * sizeof(SweepUnit), hipMemcpyHostToDevice));
+  CallTimer tk(p, "k_sweep");
+  return launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_bj, p->d_units_tmp, (int32_t)u.size(), p->shape_pipe);
 }
 
 int kr_sweep_solver_population(kr_problem *p, const int32_t *list, int32_t num, double *population) {

@@ -382,7 +382,7 @@ extern "C" ko_problem *ko_create(const ko_input *inp) {
   if (in.num_zonesets[0] <= 0 || in.num_zonesets[1] <= 0 || in.num_zonesets[2] <= 0 || in.num_groups < 1 ||
       in.num_groupsets < 1 || in.num_groups % in.num_groupsets || in.num_directions < 8 ||
       (in.num_dirsets % 8 && in.num_dirsets < 8) || in.num_directions % in.num_dirsets || in.legendre_order < 0 ||
-      in.layout < 0 || in.layout > 5) {
+      in.layout < 0 || in.layout > 5 || in.parallel_method < 0 || in.parallel_method > 1) {
     g_err = "invalid input variables";
     return nullptr;
   }
@@ -853,6 +853,58 @@ extern "C" void ko_sweep_solver(ko_problem *p, int omp) {
   }
 }
 
+/* SweepSolver with BlockJacobiComm on one rank (--pmethod bj):
+ * src/Kripke/SweepSolver.cpp:23-87, src/Kripke/ParallelComm/BlockJacobiComm.cpp:23-116,
+ * on-rank branches of src/Kripke/ParallelComm.cpp:61-263.  The communicator
+ * allocates fresh old_*_plane fields (BlockJacobiComm.cpp:31-33) whose chunks
+ * come from new double[] (src/Kripke/Core/Field.h:68): indeterminate in the
+ * reference, zero here (what a fresh allocation reads).  Kernel::kCopy copies
+ * src <- dst (src/Kripke/Kernel.h:63-80), so addSubdomain's
+ * kCopy(old_plane, plane) overwrites the live planes with the old ones. */
+extern "C" void ko_sweep_solver_bj(ko_problem *p, int omp) {
+  Scope sc(p->t_ss);
+  std::vector<std::vector<double>> *planes[3] = {&p->ipl, &p->jpl, &p->kpl};
+  for (int dim = 0; dim < 3; ++dim) /* SweepSolver.cpp:28-30 */
+    for (auto &c : *planes[dim]) kconst(c, 0.0);
+  std::vector<std::vector<double>> old[3]; /* BlockJacobiComm.cpp:31-33 */
+  for (int dim = 0; dim < 3; ++dim)
+    for (auto &c : *planes[dim]) old[dim].emplace_back(c.size(), 0.0);
+  std::vector<int> q_ids, q_dep;
+  for (int s = 0; s < p->nsdom; ++s) { /* SweepSolver.cpp:42-47 -> addSubdomain */
+    for (int dim = 0; dim < 3; ++dim) /* BlockJacobiComm.cpp:49-59, reversed kCopy */
+      for (int c = 0; c < p->nsdom; ++c) (*planes[dim])[c] = old[dim][c];
+    int nd = 0; /* postRecvs (ParallelComm.cpp:61-118): on-rank upwind deps */
+    for (int dim = 0; dim < 3; ++dim)
+      if (p->sd[s].up[dim] >= 0) nd++;
+    q_ids.push_back(s);
+    q_dep.push_back(nd);
+  }
+  /* first workRemaining() (BlockJacobiComm.cpp:68-89): postSends of the old
+   * planes of every queued subdomain, on-rank branch ParallelComm.cpp:137-162 */
+  for (size_t i = 0; i < q_ids.size(); ++i) {
+    const Sdom &S = p->sd[q_ids[i]];
+    for (int dim = 0; dim < 3; ++dim) {
+      int dn = S.down[dim];
+      if (dn < 0) continue;
+      for (size_t j = 0; j < q_ids.size(); ++j)
+        if (q_ids[j] == dn) { q_dep[j]--; break; }
+      (*planes[dim])[dn] = old[dim][q_ids[i]];
+    }
+  }
+  while (!q_ids.empty()) { /* SweepSolver.cpp:52-80; markComplete only dequeues */
+    int pick = -1;
+    for (size_t i = 0; i < q_ids.size(); ++i)
+      if (q_dep[i] == 0) { pick = (int)i; break; }
+    if (pick < 0) { g_err = "block Jacobi: unresolved dependency"; return; }
+    int s = q_ids[pick];
+    for (int dim = 0; dim < 3; ++dim)
+      if (p->sd[s].up[dim] == -1) kconst((*planes[dim])[s], 0.0);
+    ko_sweep_subdomain(p, s, omp);
+    q_ids.erase(q_ids.begin() + pick);
+    q_dep.erase(q_dep.begin() + pick);
+  }
+}
+
 /* population: src/Kripke/Kernel/Population.cpp:74-101, body :49-63 */
 extern "C" void ko_population(ko_problem *p, int omp, double *literal, double *compensated) {
   Scope sc(p->t_pop);
@@ -917,7 +969,10 @@ extern "C" void ko_iterate(ko_problem *p, int omp, double *literal, double *comp
   ko_source(p, omp);
   for (auto &c : p->rhs) kconst(c, 0.0);
   ko_lplustimes(p, omp);
-  ko_sweep_solver(p, omp);
+  if (p->in.parallel_method == 1) /* SteadyStateSolver.cpp:82-92, block_jacobi */
+    ko_sweep_solver_bj(p, omp);
+  else
+    ko_sweep_solver(p, omp);
   ko_population(p, omp, literal, compensated);
 }
 

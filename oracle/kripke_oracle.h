@@ -45,6 +45,7 @@ typedef struct ko_input {
   uint64_t seed;
   double sigt[3], sigs[3];   /* per material; overwritten by the seeded generator */
   double source_strength;
+  int32_t parallel_method;   /* 0 = sweep (SweepComm), 1 = Block Jacobi (BlockJacobiComm) */
 } ko_input;
 
 typedef struct ko_problem ko_problem;
@@ -82,6 +83,7 @@ void ko_scattering(ko_problem *p, int omp);
 void ko_source(ko_problem *p, int omp);
 void ko_sweep_subdomain(ko_problem *p, int32_t sdom, int omp);
 void ko_sweep_solver(ko_problem *p, int omp);
+void ko_sweep_solver_bj(ko_problem *p, int omp);
 /* literal = sum in the Sequential loop order (or OpenMP reduction when omp);
  * compensated = Neumaier sum of the same rounded terms. */
 void ko_population(ko_problem *p, int omp, double *literal, double *compensated);

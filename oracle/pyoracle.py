@@ -32,17 +32,18 @@ class KoInput(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("sigt", ctypes.c_double * 3), ("sigs", ctypes.c_double * 3),
         ("source_strength", ctypes.c_double),
+        ("parallel_method", ctypes.c_int32),
     ]
 
 
 def make_input(zones=(16, 16, 16), groups=32, quad=(8, 12), directions=None, legendre=4,
                layout="DGZ", dset=8, gset=1, zset=(1, 1, 1), geometry=1, seed=20261202,
                subsamples=4, sigt=(0.1, 0.0001, 0.1), sigs=(0.05, 0.00005, 0.05),
-               source_strength=1.0):
+               source_strength=1.0, pmethod="sweep"):
     """Problem description with the reference's InputVariables meaning
     (src/Kripke/InputVariables.cpp:17-46).  quad=(polar, azimuthal) selects
     the Gauss-Legendre product set, quad=None the dummy S2 set with
-    `directions` points."""
+    `directions` points; pmethod is --pmethod (sweep | bj, kripke.cpp:329-340)."""
     ki = KoInput()
     ki.nx, ki.ny, ki.nz = zones
     if quad is None:
@@ -65,6 +66,7 @@ def make_input(zones=(16, 16, 16), groups=32, quad=(8, 12), directions=None, leg
         ki.sigt[m] = sigt[m]
         ki.sigs[m] = sigs[m]
     ki.source_strength = source_strength
+    ki.parallel_method = {"sweep": 0, "bj": 1}[pmethod]
     return ki
 
 
@@ -100,6 +102,7 @@ def lib(kind="ref"):
     for fn in ("ko_ltimes", "ko_lplustimes", "ko_scattering", "ko_source", "ko_sweep_solver"):
         getattr(L, fn).argtypes = [P, ctypes.c_int]
     L.ko_sweep_subdomain.argtypes = [P, ctypes.c_int32, ctypes.c_int]
+    L.ko_sweep_solver_bj.argtypes = [P, ctypes.c_int]
     dp = ctypes.POINTER(ctypes.c_double)
     L.ko_population.argtypes = [P, ctypes.c_int, dp, dp]
     L.ko_iterate.argtypes = [P, ctypes.c_int, dp, dp]
@@ -184,8 +187,11 @@ class Oracle:
     def sweep_subdomain(self, s):
         self.L.ko_sweep_subdomain(self.h, s, self.omp)
 
-    def sweep_solver(self):
-        self.L.ko_sweep_solver(self.h, self.omp)
+    def sweep_solver(self, block_jacobi=False):
+        if block_jacobi:
+            self.L.ko_sweep_solver_bj(self.h, self.omp)
+        else:
+            self.L.ko_sweep_solver(self.h, self.omp)
 
     def population(self):
         a, b = ctypes.c_double(), ctypes.c_double()

@@ -153,6 +153,33 @@ def global_sweep_numpy(o, layout, nx, ny, nz_):
     return res
 
 
+def blockwise_sweep_numpy(o, layout):
+    """Block Jacobi on one rank: every subdomain solved alone with vacuum
+    inflow on all of its faces, from the oracle's rhs, sigt_zonal, quadrature
+    and mesh; psi per subdomain in canonical [d][g][z] order."""
+    dx, dy, dz = o.get("dx"), o.get("dy"), o.get("dz")
+    gz_first = layout in ("DGZ", "GDZ", "GZD")
+    res = {}
+    for s in range(o.num_sdoms):
+        S = o.sdom(s)
+        nd, ng, ni, nj, nk = S["nd"], S["ng"], S["ni"], S["nj"], S["nk"]
+        q = canonical(o.get("rhs", s), layout, nd, ng, ni * nj * nk).reshape(nd, ng, nk, nj, ni)
+        st = o.get("sigt_zonal", S["pr"])
+        sig = (st.reshape(ng, -1) if gz_first else st.reshape(-1, ng).T).reshape(ng, nk, nj, ni)
+        xc, yc, zc = o.get("xcos", S["dset"]), o.get("ycos", S["dset"]), o.get("zcos", S["dset"])
+        signs = tuple(int(o.get(n, S["dset"])[0]) for n in ("id", "jd", "kd"))
+        ddx = dx[S["ilo"]:S["ilo"] + ni]
+        ddy = dy[S["jlo"]:S["jlo"] + nj]
+        ddz = dz[S["klo"]:S["klo"] + nk]
+        psi = np.zeros((nd, ng, nk, nj, ni))
+        for d in range(nd):
+            for g in range(ng):
+                psi[d, g] = dd_sweep_linear_system(q[d, g], sig[g], 2.0 * xc[d] / ddx, 2.0 * yc[d] / ddy,
+                                                   2.0 * zc[d] / ddz, signs)
+        res[s] = psi.reshape(nd, ng, nk * nj * ni)
+    return res
+
+
 def population_fsum(o, layout):
     terms = []
     for s in range(o.num_sdoms):

@@ -162,6 +162,69 @@ def test_sweep_solver_pipeline(ctx, layout):
     _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_sweep_solver_block_jacobi(ctx, layout):
+    """SweepSolver with block_jacobi (--pmethod bj): every subdomain swept
+    with zero incoming faces.  psi and the outgoing faces left in each
+    subdomain's planes against the oracle; the planes start from random
+    values, which must not leak in."""
+    kw = kwargs_for(layout, zset=(2, 2, 1), groups=4, zones=(8, 6, 10))
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(24)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    for name in ("i_plane", "j_plane", "k_plane"):
+        seeded_fill(hp, [of, orf], name, hp.num_sdoms, rng)
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)), block_jacobi=True)
+    of.sweep_solver(block_jacobi=True)
+    orf.sweep_solver(block_jacobi=True)
+    for name in ("psi", "i_plane", "j_plane", "k_plane"):
+        _assert_chunks(hp, of, orf, name, hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
+
+
+@pytest.mark.parametrize("layout", ["DGZ", "ZGD"])
+def test_sweep_solver_block_jacobi_partial_list(ctx, layout):
+    """A partial, unordered list: listed subdomains as in the full sweep,
+    unlisted ones keep psi and end with zero planes (the old planes)."""
+    kw = kwargs_for(layout, zset=(2, 2, 1), groups=4, zones=(8, 6, 10))
+    hp, of, _ = _trio(ctx, **kw)
+    rng = np.random.default_rng(25)
+    seeded_fill(hp, [of], "rhs", hp.num_sdoms, rng, offset=1.0)
+    seeded_fill(hp, [of], "psi", hp.num_sdoms, rng)
+    for name in ("i_plane", "j_plane", "k_plane"):
+        seeded_fill(hp, [of], name, hp.num_sdoms, rng)
+    psi0 = [hp.get("psi", s) for s in range(hp.num_sdoms)]
+    listed = [5, 0, 3]
+    hp.kernels().sweep_solver(listed, block_jacobi=True)
+    of.sweep_solver(block_jacobi=True)
+    for s in range(hp.num_sdoms):
+        if s in listed:
+            a, b = hp.get("psi", s), of.get("psi", s)
+            assert (np.array_equal(a, b) if layout not in ZFAST else rel_normwise(a, b) <= TOL_SCAN), s
+        else:
+            assert np.array_equal(hp.get("psi", s), psi0[s]), s
+            for name in ("i_plane", "j_plane", "k_plane"):
+                assert not np.any(hp.get(name, s)), (name, s)
+    with pytest.raises(Exception):
+        hp.kernels().sweep_solver([0, 0], block_jacobi=True)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("layout", ["DGZ", "DZG", "GZD"])
+def test_iterations_block_jacobi(ctx, layout, mode):
+    """Source iterations with --pmethod bj (2x2x2 zone sets): population and
+    phi per iteration against the oracle (1e-12), both modes."""
+    kw = kwargs_for(layout, zones=(12, 8, 8), zset=(2, 2, 2), groups=8, quad=(4, 4), legendre=2, pmethod="bj")
+    hp, of, orf = _trio(ctx, **kw)
+    hp.init_psi()
+    orf.init_psi()
+    for it in range(4):
+        pg = hp.iterate(mode)
+        _, comp = orf.iterate()
+        assert abs(pg - comp) <= TOL_REF * abs(comp), "iteration %d population %.17g vs %.17g" % (it, pg, comp)
+        norm, _ = phi_parity(hp, orf, hp.num_pr, layout, 9, 8, 6 * 4 * 4)
+        assert norm <= TOL_REF, "iteration %d phi normwise %.3e" % (it, norm)
+
+
 @pytest.mark.parametrize("layout", ZFAST)
 def test_sweep_kpatches(ctx, layout):
     """A 64 x 40 x 64 subdomain: 32 lanes per row (2 zones each) x 64 k-rows
