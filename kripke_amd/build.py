@@ -23,7 +23,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-value",
 ]
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result", "-I/opt/rocm/include"]
 
 
 def _run(cmd):
@@ -48,8 +48,14 @@ def build(verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, LIB)
+    # the kripke command, linked against the library next to it
+    drv = os.path.join(HERE, "kripke")
+    _run([CXX] + [f for f in CXX_FLAGS if f != "-fPIC"] + ["-o", drv + ".tmp", os.path.join(CSRC, "driver", "kripke.cpp"),
+          "-L" + HERE, "-lkripke_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

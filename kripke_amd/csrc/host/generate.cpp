@@ -231,7 +231,8 @@ void Kripke::generateProblem(DataStore &ds, const InputVariables &in, kr_context
   const bool dev = ctx != nullptr;
   const int L = in.layout;
   using I = Idx;
-  ds.addVariable("timing", new Timing());
+  if (!ds.isVariableDefined("timing")) ds.addVariable("timing", new Timing());  // a caller may preset it
+  KRIPKE_TIMER(ds, Generate);
   ds.newVariable<ArchLayout>("al").layout = L;
 
   // Decomp: one rank, P=Q=1 ranks (Decomp.cpp:23-56)

@@ -4,6 +4,7 @@
 // SteadyStateSolver.h,SweepSolver.h}; the bodies call the C-ABI.
 #pragma once
 #include <chrono>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -33,20 +34,46 @@ struct MaterialData : public Core::BaseVar {
   double source_strength = 1.0;
 };
 
-// Named accumulating timers with the reference names (src/Kripke/Timing.h).
+// Named accumulating wall-clock timers with the reference names and
+// semantics (src/Kripke/Timing.h:25-115, Timing.cpp): the count rises at
+// start, the elapsed time accumulates at stop.  Off unless `enabled`.  With
+// `sync` the stop first waits for the device, so a timer covers the work its
+// calls enqueued (the reference's calls are synchronous).  Every timed block
+// is also a roctx range of the same name (the reference's Caliper
+// annotation, Timing.h:91-107).
 struct Timing : public Core::BaseVar {
   struct T {
     double total = 0;
     int64_t count = 0;
   };
-  std::vector<std::pair<std::string, T>> timers;
-  T &get(const std::string &n) {
-    for (auto &kv : timers)
-      if (kv.first == n) return kv.second;
-    timers.push_back({n, T{}});
-    return timers.back().second;
+  bool enabled = false, sync = false;
+  kr_context *ctx = nullptr;
+  std::map<std::string, T> timers;
+  T &get(const std::string &n) { return timers[n]; }
+  double getTotal(const std::string &n) const {
+    auto it = timers.find(n);
+    return it == timers.end() ? 0.0 : it->second.total;
   }
+  size_t getCount(const std::string &n) const {
+    auto it = timers.find(n);
+    return it == timers.end() ? 0 : (size_t)it->second.count;
+  }
+  // Timers table and the machine-readable lines (Timing.cpp:45-95)
+  void print() const;
 };
+
+// KRIPKE_TIMER analogue: times the enclosing block when timing is enabled.
+class BlockTimer {
+ public:
+  BlockTimer(Core::DataStore &ds, const char *name);
+  ~BlockTimer();
+
+ private:
+  Timing *m_t = nullptr;
+  const char *m_name;
+  std::chrono::steady_clock::time_point m_t0;
+};
+#define KRIPKE_TIMER(DS, NAME) Kripke::BlockTimer kr_block_timer_##NAME(DS, #NAME)
 
 void generateProblem(Core::DataStore &ds, const InputVariables &in, kr_context *ctx);
 
@@ -72,6 +99,10 @@ void scatteringOverwrite(Core::DataStore &ds);
 void SweepSolver(Core::DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline, bool block_jacobi = false);
 // One iteration of src/Kripke/SteadyStateSolver.cpp:44-104.
 double SteadyStateIteration(Core::DataStore &ds, int mode, bool block_jacobi = false);
+// src/Kripke/SteadyStateSolver.cpp:24-115: psi = 0, max_iter iterations,
+// the reference's per-iteration particle-count lines on stdout when
+// `verbose`.  Returns 0.
+int SteadyStateSolver(Core::DataStore &ds, size_t max_iter, bool block_jacobi, int mode, bool verbose = true);
 
 // Thrown on any C-ABI failure inside the host mirror (KRIPKE_ABORT analogue).
 struct KripkeError : public std::runtime_error {

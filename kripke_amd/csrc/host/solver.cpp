@@ -1,7 +1,10 @@
 // Kripke::Kernel entry points, SweepSolver and the steady-state iteration
 // of the host mirror, driving the C-ABI (src/Kripke/Kernel/*.cpp,
 // src/Kripke/SweepSolver.cpp, src/Kripke/SteadyStateSolver.cpp).
+#include <cstdio>
 #include <cstring>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include "kripke_host.h"
 
@@ -12,6 +15,45 @@ namespace {
 kr_problem *kp(DataStore &ds) { return ds.getVariable<KernelProblem>("kr_problem").prob; }
 }  // namespace
 
+BlockTimer::BlockTimer(DataStore &ds, const char *name) : m_name(name) {
+  if (!ds.isVariableDefined("timing")) return;
+  Timing &t = ds.getVariable<Timing>("timing");
+  if (!t.enabled) return;
+  m_t = &t;
+  roctxRangePushA(name);
+  t.get(name).count++;
+  m_t0 = std::chrono::steady_clock::now();
+}
+
+BlockTimer::~BlockTimer() {
+  if (!m_t) return;
+  if (m_t->sync && m_t->ctx) kr_synchronize(m_t->ctx);
+  m_t->get(m_name).total += std::chrono::duration<double>(std::chrono::steady_clock::now() - m_t0).count();
+  roctxRangePop();
+}
+
+void Timing::print() const {
+  // std::map keeps the names sorted, as Timing.cpp:51-57 sorts them
+  printf("\nTimers\n");
+  printf("======\n\n");
+  printf("  %-16s  %12s  %12s\n", "Timer", "Count", "Seconds");
+  printf("  ----------------  ------------  ------------\n");
+  for (const auto &kv : timers) printf("  %-16s  %12d  %12.5lf\n", kv.first.c_str(), (int)kv.second.count, kv.second.total);
+  printf("\nTIMER_NAMES:");
+  bool first = true;
+  for (const auto &kv : timers) {
+    printf("%s%s", first ? "" : ",", kv.first.c_str());
+    first = false;
+  }
+  printf("\nTIMER_DATA:");
+  first = true;
+  for (const auto &kv : timers) {
+    printf("%s%lf", first ? "" : ",", kv.second.total);
+    first = false;
+  }
+  printf("\n");
+}
+
 void Kernel::kConst(Field<double> &field, SdomId sdom_id, double value) {
   check(kr_kconst(field.context(), field.device(sdom_id), (int64_t)field.size(sdom_id), value), "kConst");
 }
@@ -20,22 +62,43 @@ void Kernel::kConst(Field<double> &field, double value) {
   for (SdomId s : field.getWorkList()) kConst(field, s, value);
 }
 
-void Kernel::LTimes(DataStore &ds) { check(kr_ltimes(kp(ds), 1), "LTimes"); }
-void Kernel::LPlusTimes(DataStore &ds) { check(kr_lplustimes(kp(ds), 1), "LPlusTimes"); }
-void Kernel::scattering(DataStore &ds) { check(kr_scattering(kp(ds), 1), "scattering"); }
-void Kernel::LTimesOverwrite(DataStore &ds) { check(kr_ltimes(kp(ds), 0), "LTimes"); }
-void Kernel::LPlusTimesOverwrite(DataStore &ds) { check(kr_lplustimes(kp(ds), 0), "LPlusTimes"); }
-void Kernel::scatteringOverwrite(DataStore &ds) { check(kr_scattering(kp(ds), 0), "scattering"); }
+void Kernel::LTimes(DataStore &ds) {
+  KRIPKE_TIMER(ds, LTimes);
+  check(kr_ltimes(kp(ds), 1), "LTimes");
+}
+void Kernel::LPlusTimes(DataStore &ds) {
+  KRIPKE_TIMER(ds, LPlusTimes);
+  check(kr_lplustimes(kp(ds), 1), "LPlusTimes");
+}
+void Kernel::scattering(DataStore &ds) {
+  KRIPKE_TIMER(ds, Scattering);
+  check(kr_scattering(kp(ds), 1), "scattering");
+}
+void Kernel::LTimesOverwrite(DataStore &ds) {
+  KRIPKE_TIMER(ds, LTimes);
+  check(kr_ltimes(kp(ds), 0), "LTimes");
+}
+void Kernel::LPlusTimesOverwrite(DataStore &ds) {
+  KRIPKE_TIMER(ds, LPlusTimes);
+  check(kr_lplustimes(kp(ds), 0), "LPlusTimes");
+}
+void Kernel::scatteringOverwrite(DataStore &ds) {
+  KRIPKE_TIMER(ds, Scattering);
+  check(kr_scattering(kp(ds), 0), "scattering");
+}
 
 void Kernel::source(DataStore &ds) {
+  KRIPKE_TIMER(ds, Source);
   check(kr_source(kp(ds), ds.getVariable<MaterialData>("materials").source_strength), "source");
 }
 
 void Kernel::sweepSubdomain(DataStore &ds, SdomId sdom_id) {
+  KRIPKE_TIMER(ds, SweepSubdomain);
   check(kr_sweep_subdomain(kp(ds), sdom_id), "sweepSubdomain");
 }
 
 double Kernel::population(DataStore &ds) {
+  KRIPKE_TIMER(ds, Population);
   double part = 0.0;
   check(kr_population(kp(ds), &part), "population");
   return part;  // Comm::allReduceSumDouble is the identity on one rank
@@ -46,6 +109,7 @@ double Kernel::population(DataStore &ds) {
 // (BlockJacobiComm.cpp:23-116).  pipeline=false replays the reference queue
 // on the host; pipeline=true hands the whole list to one device call.
 void Kripke::SweepSolver(DataStore &ds, const std::vector<SdomId> &sdom_list, bool pipeline, bool block_jacobi) {
+  KRIPKE_TIMER(ds, SweepSolver);
   if (pipeline) {
     if (block_jacobi)
       check(kr_sweep_solver_bj(kp(ds), sdom_list.data(), (int32_t)sdom_list.size()), "SweepSolver");
@@ -139,8 +203,32 @@ double Kripke::SteadyStateIteration(DataStore &ds, int mode, bool block_jacobi) 
       return Kernel::population(ds);
     }
     double part = 0.0;  // SweepSolver + population in one pass over psi
-    check(kr_sweep_solver_population(kp(ds), all.data(), (int32_t)all.size(), &part), "SweepSolver");
+    {
+      KRIPKE_TIMER(ds, SweepSolver);
+      check(kr_sweep_solver_population(kp(ds), all.data(), (int32_t)all.size(), &part), "SweepSolver");
+    }
     return part;
   }
   return Kernel::population(ds);
+}
+
+int Kripke::SteadyStateSolver(DataStore &ds, size_t max_iter, bool block_jacobi, int mode, bool verbose) {
+  KRIPKE_TIMER(ds, Solve);
+  if (verbose) {
+    printf("\n");
+    printf("Steady State Solve\n");
+    printf("==================\n\n");
+  }
+  Kernel::kConst(ds.getVariable<Field<double>>("psi"), 0.0);  // SteadyStateSolver.cpp:39
+  double part_last = 0.0;
+  for (size_t iter = 0; iter < max_iter; ++iter) {
+    const double part = SteadyStateIteration(ds, mode, block_jacobi);
+    if (verbose) {
+      printf("  iter %d: particle count=%e, change=%e\n", (int)iter, part, (part - part_last) / part);
+      fflush(stdout);
+    }
+    part_last = part;
+  }
+  if (verbose) printf("  Solver terminated\n");
+  return 0;
 }
