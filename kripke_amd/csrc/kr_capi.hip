@@ -182,6 +182,46 @@ void append_units(std::vector<SweepUnit> &u, int sdom, int32_t pgs, int32_t R) {
   for (int32_t b = 0; b < pgs; b += R) u.push_back(SweepUnit{sdom, b, std::min(R, pgs - b), 0});
 }
 
+// Dispatch order of one launch's units [begin, begin + n).  Units that read
+// the same sigma_t rows in the same order (same sigma_t chunk, same group
+// sequence: with the group index fastest it is fixed by the first pair
+// modulo the group count, otherwise by the first pair) are placed back to
+// back on one XCD: workgroups go to the 8 XCDs round-robin, so slot j*8 + x
+// runs on XCD x, and sorted unit i is given XCD i / ceil(n/8) and the next
+// free slot there.  Such units start together and stream the same rows
+// through that XCD's L2.  Units of a launch are independent, so only speed
+// depends on this order.
+void order_units_xcd(const kr_problem *p, const SweepShape &sh, std::vector<SweepUnit> &u, size_t begin, size_t n) {
+  if (n < 2) return;
+  struct Key {
+    uintptr_t sig;
+    int64_t gseq;
+    int32_t cnt;
+    size_t idx;
+  };
+  std::vector<Key> k(n);
+  for (size_t i = 0; i < n; ++i) {
+    const SweepUnit &U = u[begin + i];
+    const DevSdom &S = p->h_sd[U.sdom];
+    const int64_t p0 = (int64_t)U.pg_begin * sh.Q;
+    k[i] = Key{(uintptr_t)S.sig, sh.inner_is_g ? p0 % S.ng : p0, U.pg_count, i};
+  }
+  std::sort(k.begin(), k.end(), [](const Key &a, const Key &b) {
+    if (a.sig != b.sig) return a.sig < b.sig;
+    if (a.gseq != b.gseq) return a.gseq < b.gseq;
+    if (a.cnt != b.cnt) return a.cnt < b.cnt;
+    return a.idx < b.idx;
+  });
+  const size_t q = n / 8, rem = n % 8;
+  std::vector<SweepUnit> out(n);
+  size_t i = 0;
+  for (size_t x = 0; x < 8; ++x) {
+    const size_t cx = q + (x < rem ? 1 : 0);
+    for (size_t j = 0; j < cx; ++j) out[j * 8 + x] = u[begin + k[i++].idx];
+  }
+  std::copy(out.begin(), out.end(), u.begin() + begin);
+}
+
 // The sweep kernels address each unit's slice of psi/rhs, the planes and
 // sigma_t through buffer resources with 32-bit byte offsets: every unit's span
 // (from its first pair's first element to its last pair's last element) must
@@ -566,6 +606,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
       p->ref_unit_begin.push_back((int32_t)uref.size());
       append_units(uref, s, pgs, R);
       p->ref_unit_count.push_back((int32_t)uref.size() - p->ref_unit_begin.back());
+      order_units_xcd(p, p->shape_ref, uref, p->ref_unit_begin.back(), p->ref_unit_count.back());
     }
     std::vector<int32_t> all(p->nsdom);
     for (int s = 0; s < p->nsdom; ++s) all[s] = s;
@@ -576,6 +617,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
       p->pipe_level_begin.push_back((int32_t)upipe.size());
       for (int32_t s : lv) append_units(upipe, s, pgs, Rl);
       p->pipe_level_count.push_back((int32_t)upipe.size() - p->pipe_level_begin.back());
+      order_units_xcd(p, p->shape_pipe, upipe, p->pipe_level_begin.back(), p->pipe_level_count.back());
     }
   }
 
@@ -717,6 +759,7 @@ static int sweep_solver_impl(kr_problem *p, const int32_t *list, int32_t num, do
     lb.push_back((int32_t)u.size());
     for (int32_t s : lv) append_units(u, s, pgs, Rl);
     lc.push_back((int32_t)u.size() - lb.back());
+    order_units_xcd(p, p->shape_pipe, u, lb.back(), lc.back());
   }
   if ((rc = check_unit_spans(p, u, p->shape_pipe))) return rc;
   KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
@@ -771,6 +814,7 @@ int kr_sweep_solver_bj(kr_problem *p, const int32_t *list, int32_t num) {
   const int32_t R = pick_R((int64_t)pgs * num, p->shape_pipe);
   std::vector<SweepUnit> u;
   for (int32_t i = 0; i < num; ++i) append_units(u, list[i], pgs, R);
+  order_units_xcd(p, p->shape_pipe, u, 0, u.size());
   if ((rc = check_unit_spans(p, u, p->shape_pipe))) return rc;
   KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
   if ((int32_t)u.size() > p->units_tmp_cap) {
@@ -954,6 +998,7 @@ static int build_pipe_schedule(kr_problem *p, const SweepShape &sh, int r_scale_
     lbeg.push_back((int32_t)units.size());
     for (int32_t s : lv) append_units(units, s, pgs, (int32_t)R);
     lcnt.push_back((int32_t)units.size() - lbeg.back());
+    order_units_xcd(p, sh, units, lbeg.back(), lcnt.back());
   }
   return check_unit_spans(p, units, sh);
 }
