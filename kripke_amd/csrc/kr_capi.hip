@@ -1,6 +1,8 @@
 // C-ABI layer: contexts, device memory, problem registration and the
 // Kernel:: entry points declared in include/kripke_amd.h.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -1063,6 +1065,7 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
     hipEvent_t e0, e1;
     KR_HIP_CHECK(hipEventCreate(&e0));
     KR_HIP_CHECK(hipEventCreate(&e1));
+    const bool verbose = getenv("KR_AUTOTUNE_VERBOSE") != nullptr;  // candidate timings on stderr
     double best_ms = 1e300;
     int rc = KR_OK;
     for (TuneChoice &c : cand) {
@@ -1083,6 +1086,9 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
         if (rep > 0) ms = std::min(ms, (double)t);
       }
       c.ms = ms;
+      if (verbose)
+        fprintf(stderr, "kr_problem_autotune: v_max %d cap_div %d R %d%%: %.3f ms\n", c.v_max, c.cap_div,
+                c.r_scale_pct, ms);
       if (!rc && ms < best_ms) {
         best_ms = ms;
         best = c;

@@ -97,32 +97,60 @@ constexpr int DPP_WAVE_SHR1 = 0x138;
 constexpr int DPP_ROW_BCAST15 = 0x142;
 constexpr int DPP_ROW_BCAST31 = 0x143;
 
-template <int CTRL>
+// Two wait states between the VALU write of a value and a DPP read of it,
+// as an explicit s_nop tied to the values (the asm "rewrites" them, so every
+// DPP read of them comes after it) rather than whatever instructions the
+// scheduler happens to place there.
+__device__ __forceinline__ void dpp_wait(double &a, double &b) { asm volatile("s_nop 1" : "+v"(a), "+v"(b)); }
+__device__ __forceinline__ void dpp_wait(double &a) { asm volatile("s_nop 1" : "+v"(a)); }
+
+// DPP move of a double in which lanes without a source lane, and rows outside
+// ROWS, keep `fill` (the move does not write them).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_or(double v, double fill) {
+  const long long s = __double_as_longlong(v), f = __double_as_longlong(fill);
+  const int lo = __builtin_amdgcn_update_dpp((int)f, (int)s, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(f >> 32), (int)(s >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One scan step: compose with the map CTRL brings in.  ID: a lane that gets
+// no map composes with the identity (A = 1, B = 0) instead of zero.
+template <int CTRL, int ROWS, bool ID>
 __device__ __forceinline__ void compose_from(double &A, double &B) {
-  const double Ap = dpp<CTRL>(A);
-  const double Bp = dpp<CTRL>(B);
+  dpp_wait(A, B);
+  const double Ap = ID ? dpp_or<CTRL, ROWS>(A, 1.0) : dpp<CTRL>(A);
+  const double Bp = ID ? dpp_or<CTRL, ROWS>(B, 0.0) : dpp<CTRL>(B);
   B = __fma_rn(A, Bp, B);
   A = A * Ap;
 }
 
 // Inclusive scan of affine maps f -> A f + B over the W lanes of a row group,
 // lane 0 first.  Lane 0's map is constant (A = 0: it already holds the row's
-// incoming face), so every lane whose partial composite reaches back to lane 0
-// has A = 0 and is unchanged by whatever finite map it is composed with next;
-// the shifts therefore need no masking at row-group or row boundaries.
+// incoming face), so a lane whose partial composite reaches back to lane 0
+// has A = 0 and is unchanged by whatever finite map it is composed with next.
+// Within 16-lane rows (W <= 16) every lane's partial composite reaches lane 0
+// by the time a shift runs past it, so out-of-row lanes may read zero.  A row
+// group over several rows (W = 32, 64) starts rows 1.. mid-group: there a
+// lane without a source must compose with the identity (with zeros, lane 16
+// would drop the first 16 lanes' composite), and the cross-row broadcasts are
+// masked as in a two-level scan (lane 15 into rows 1 and 3, then lane 31 into
+// rows 2 and 3) so no segment is composed twice.
 template <int W>
 __device__ __forceinline__ void scan_maps(double &A, double &B) {
-  if constexpr (W > 1) compose_from<DPP_ROW_SHR + 1>(A, B);
-  if constexpr (W > 2) compose_from<DPP_ROW_SHR + 2>(A, B);
-  if constexpr (W > 4) compose_from<DPP_ROW_SHR + 4>(A, B);
-  if constexpr (W > 8) compose_from<DPP_ROW_SHR + 8>(A, B);
-  if constexpr (W > 16) compose_from<DPP_ROW_BCAST15>(A, B);
-  if constexpr (W > 32) compose_from<DPP_ROW_BCAST31>(A, B);
+  constexpr bool ID = W > 16;
+  if constexpr (W > 1) compose_from<DPP_ROW_SHR + 1, 0xF, ID>(A, B);
+  if constexpr (W > 2) compose_from<DPP_ROW_SHR + 2, 0xF, ID>(A, B);
+  if constexpr (W > 4) compose_from<DPP_ROW_SHR + 4, 0xF, ID>(A, B);
+  if constexpr (W > 8) compose_from<DPP_ROW_SHR + 8, 0xF, ID>(A, B);
+  if constexpr (W > 16) compose_from<DPP_ROW_BCAST15, 0xA, ID>(A, B);
+  if constexpr (W > 32) compose_from<DPP_ROW_BCAST31, 0xC, ID>(A, B);
 }
 
 // Value of the lane one place back in the row group (lane 0: `first`).
 template <int W>
 __device__ __forceinline__ double lane_back(double v, double first, int l) {
+  dpp_wait(v);
   const double u = W <= 16 ? dpp<DPP_ROW_SHR + 1>(v) : dpp<DPP_WAVE_SHR1>(v);
   return l == 0 ? first : u;
 }

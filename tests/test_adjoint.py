@@ -145,6 +145,24 @@ def test_sweep_transpose(ctx, layout, zones):
     _check_dot((y, Hx), (Hty, x))
 
 
+@pytest.mark.parametrize("layout", ["DGZ", "GDZ"])
+@pytest.mark.parametrize("zones,zset,quad", [((32, 24, 32), (1, 1, 2), (4, 8)), ((32, 12, 32), (1, 2, 1), (4, 8)),
+                                             ((16, 16, 16), (2, 1, 1), (8, 8)), ((128, 6, 8), (1, 1, 2), (4, 4))])
+def test_sweep_transpose_row_widths(ctx, layout, zones, zset, quad):
+    """The adjoint sweep for rows of 16 and 8 lanes, with k-patches and zone-set
+    pipelines."""
+    hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=2, zones=zones, quad=quad, legendre=1))
+    k = hp.kernels()
+    rng = np.random.default_rng(46)
+    x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.sweep_solver(list(range(hp.num_sdoms)))
+    Hx = _get(hp, "psi", hp.num_sdoms)
+    y = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    k.sweep_solver_adjoint()
+    Hty = _get(hp, "rhs", hp.num_sdoms)
+    _check_dot((y, Hx), (Hty, x))
+
+
 # --------------------------------------------------------------------------
 # gradients
 # --------------------------------------------------------------------------

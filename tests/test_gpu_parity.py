@@ -245,6 +245,47 @@ def test_sweep_kpatches(ctx, layout):
     _assert_chunks(hp, of, None, "psi", hp.num_sdoms, exact=False)
 
 
+# Row widths of the row-scan sweep: (zones, zone sets, GL quadrature); a row
+# of ni zones takes W = ni/2 lanes (2 zones per lane).  W = 32 and 64 span
+# several 16-lane DPP rows, where the scan crosses rows.
+ROW_WIDTH_CASES = [
+    ((32, 24, 16), (1, 1, 1), (4, 8)),   # W=16, one k-patch of 16 rows
+    ((32, 12, 32), (1, 1, 1), (4, 8)),   # W=16, k-patches of 8 rows: k-face hand-off through the own plane
+    ((16, 16, 16), (1, 2, 1), (8, 8)),   # W=8, pipeline with face hand-off
+    ((64, 8, 8), (1, 1, 1), (4, 4)),     # W=32: two DPP rows per zone row
+    ((128, 6, 8), (1, 1, 2), (4, 4)),    # W=64: four DPP rows per zone row, pipeline along k
+]
+
+
+@pytest.mark.parametrize("case", range(len(ROW_WIDTH_CASES)))
+@pytest.mark.parametrize("layout", ZFAST)
+def test_sweep_row_widths(ctx, layout, case):
+    """Pipelined SweepSolver, the fused population and Kernel::sweepSubdomain
+    for row widths of 8 to 64 lanes."""
+    zones, zset, quad = ROW_WIDTH_CASES[case]
+    kw = kwargs_for(layout, zones=zones, zset=zset, quad=quad, groups=2, legendre=1)
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(26 + case)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    k = hp.kernels()
+    sdoms = list(range(hp.num_sdoms))
+    k.sweep_solver(sdoms)
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=False)
+    pop = k.sweep_solver_population(sdoms)
+    _assert_chunks(hp, of, None, "psi", hp.num_sdoms, exact=False)
+    _, comp = orf.population()
+    assert abs(pop - comp) <= TOL_REF * abs(comp)
+    for name in ("i_plane", "j_plane", "k_plane"):
+        seeded_fill(hp, [of], name, hp.num_sdoms, rng)
+    for s in sdoms:
+        k.sweep_subdomain(s)
+        of.sweep_subdomain(s)
+    for name in ("psi", "i_plane", "j_plane", "k_plane"):
+        _assert_chunks(hp, of, None, name, hp.num_sdoms, exact=False)
+
+
 @pytest.mark.parametrize("layout", ["DGZ", "DZG", "GZD"])
 def test_sweep_many_directions(ctx, layout):
     """512 directions (64 per direction set) on a 32^3 subdomain: the sweep's
