@@ -374,6 +374,11 @@ constexpr int SG_PAD = 64;
 constexpr int GF_ZT = 16;  // zones per lane, group-fastest kernel
 constexpr int GF_HB = 8;   // source groups per scalar load, group-fastest kernel
 constexpr int MF_ZS = 252; // zones per supertile, moment-fastest kernel
+// Dense launches cap their grids (the kernels loop over the rest): with an
+// uncoupled sigs they return at once, and the launch then costs few blocks.
+// The moment-fastest kernel runs best with fewer, longer blocks.
+constexpr int64_t DENSE_MAX_BLOCKS = 16384;
+constexpr int64_t DENSE_MAX_BLOCKS_MFAST = 4096;
 
 static int64_t tab_doubles(int64_t L1, int64_t M) { return (L1 + 1 + M + 1) / 2 + 1; }
 
@@ -495,18 +500,19 @@ __global__ __launch_bounds__(256) void k_scatter_dense(const DevPR *__restrict__
   if (!sigs_coupled(flags)) return;
   KR_DENSE_PAIR
   const int noc = (Po.ng + SG_OC - 1) / SG_OC;
-  const int nm = blockIdx.y / noc;
-  const int o0 = (blockIdx.y - nm * noc) * SG_OC;
-  if (nm >= M || o0 >= Po.ng) return;
-  const int64_t nl = m2l[nm];
-  if (nl < 0 || nl >= L1) return;
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = z < Po.nz;
-  const int zc = live ? z : 0;
-  double acc[SG_OC];
-  dense_lane(acc, Wr + zc, wstride, live, T, gh_lo * G + go_lo + o0, nl, nmat, L1, G,
-             gp(inb) + nm * Pi.ma + zc * Pi.mz, Pi.mg, Pi.ng);
-  if (live) dense_store(acc, out, nm * Po.ma + (int64_t)z * Po.mz, o0, Po.ng, Po.mg, accumulate);
+  for (int yb = blockIdx.y; yb < M * noc; yb += gridDim.y) {  // the grid is capped in y
+    const int nm = yb / noc;
+    const int o0 = (yb - nm * noc) * SG_OC;
+    const int64_t nl = m2l[nm];
+    if (nl < 0 || nl >= L1) continue;
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = z < Po.nz;
+    const int zc = live ? z : 0;
+    double acc[SG_OC];
+    dense_lane(acc, Wr + zc, wstride, live, T, gh_lo * G + go_lo + o0, nl, nmat, L1, G,
+               gp(inb) + nm * Pi.ma + zc * Pi.mz, Pi.mg, Pi.ng);
+    if (live) dense_store(acc, out, nm * Po.ma + (int64_t)z * Po.mz, o0, Po.ng, Po.mg, accumulate);
+  }
 }
 
 // Moment fastest: a wave covers zpw = 64 / mc zones x mc moments of one
@@ -523,36 +529,40 @@ __global__ __launch_bounds__(256) void k_scatter_dense_mfast(const DevPR *__rest
                                                              const double *__restrict__ T,
                                                              const int32_t *__restrict__ tab, int64_t nmat,
                                                              int64_t L1, int64_t G, int src, int dst, int accumulate,
-                                                             const int32_t *__restrict__ flags, int nocg) {
+                                                             const int32_t *__restrict__ flags, int nocg,
+                                                             int64_t nblk) {
   if (!sigs_coupled(flags)) return;
   KR_DENSE_PAIR
   const int64_t nbt = L1 * nocg;
-  const int64_t bi = blockIdx.x;
-  const int64_t xi = bi >> 3;
-  const int64_t st = (xi / nbt) * 8 + (bi & 7);
-  const int k = (int)(xi % nbt);
-  const int nl = k / nocg;
-  const int o0 = (k - nl * nocg) * SG_OC;
-  const int64_t zs0 = st * MF_ZS;
-  if (zs0 >= Po.nz || o0 >= Po.ng) return;
-  const int64_t zs1 = (zs0 + MF_ZS < Po.nz) ? zs0 + MF_ZS : Po.nz;
-  const int start = tab[nl], cnt = tab[nl + 1] - start;
-  if (cnt <= 0) return;
-  const int mc = cnt < 64 ? cnt : 64, nmc = (cnt + 63) / 64, zpw = 64 / mc;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int zl = lane / mc, ml = lane - zl * mc;
-  const int64_t nwork = ((zs1 - zs0 + zpw - 1) / zpw) * nmc;
-  for (int64_t w = wave; w < nwork; w += 4) {
-    const int64_t zg = w / nmc;
-    const int mk = (int)(w - zg * nmc) * 64 + ml;
-    const int64_t z = zs0 + zg * zpw + zl;
-    const bool live = zl < zpw && z < zs1 && mk < cnt;
-    const int nm = tab[L1 + 1 + start + (live ? mk : 0)];
-    const int64_t zc = live ? z : 0;
-    double acc[SG_OC];
-    dense_lane(acc, Wr + zc, wstride, live, T, gh_lo * G + go_lo + o0, nl, nmat, L1, G,
-               gp(inb) + nm * Pi.ma + zc * Pi.mz, Pi.mg, Pi.ng);
-    if (live) dense_store(acc, out, nm * Po.ma + z * Po.mz, o0, Po.ng, Po.mg, accumulate);
+  // the grid is capped: a block takes work blocks bi, bi + gridDim.x, ...
+  // (gridDim.x is a multiple of 8, so each keeps its XCD)
+  for (int64_t bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
+    const int64_t xi = bi >> 3;
+    const int64_t st = (xi / nbt) * 8 + (bi & 7);
+    const int k = (int)(xi % nbt);
+    const int nl = k / nocg;
+    const int o0 = (k - nl * nocg) * SG_OC;
+    const int64_t zs0 = st * MF_ZS;
+    if (zs0 >= Po.nz || o0 >= Po.ng) continue;
+    const int64_t zs1 = (zs0 + MF_ZS < Po.nz) ? zs0 + MF_ZS : Po.nz;
+    const int start = tab[nl], cnt = tab[nl + 1] - start;
+    if (cnt <= 0) continue;
+    const int mc = cnt < 64 ? cnt : 64, nmc = (cnt + 63) / 64, zpw = 64 / mc;
+    const int zl = lane / mc, ml = lane - zl * mc;
+    const int64_t nwork = ((zs1 - zs0 + zpw - 1) / zpw) * nmc;
+    for (int64_t w = wave; w < nwork; w += 4) {
+      const int64_t zg = w / nmc;
+      const int mk = (int)(w - zg * nmc) * 64 + ml;
+      const int64_t z = zs0 + zg * zpw + zl;
+      const bool live = zl < zpw && z < zs1 && mk < cnt;
+      const int nm = tab[L1 + 1 + start + (live ? mk : 0)];
+      const int64_t zc = live ? z : 0;
+      double acc[SG_OC];
+      dense_lane(acc, Wr + zc, wstride, live, T, gh_lo * G + go_lo + o0, nl, nmat, L1, G,
+                 gp(inb) + nm * Pi.ma + zc * Pi.mz, Pi.mg, Pi.ng);
+      if (live) dense_store(acc, out, nm * Po.ma + z * Po.mz, o0, Po.ng, Po.mg, accumulate);
+    }
   }
 }
 
@@ -571,16 +581,16 @@ __global__ __launch_bounds__(256) void k_scatter_dense_gfast(const DevPR *__rest
   if (!sigs_coupled(flags)) return;
   KR_DENSE_PAIR
   const int noc = (Po.ng + 63) / 64;
-  const int nm = blockIdx.y / noc;
-  const int o0 = (blockIdx.y - nm * noc) * 64;
-  if (nm >= M || o0 >= Po.ng) return;
-  const int64_t nl = m2l[nm];
-  if (nl < 0 || nl >= L1) return;
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int o = o0 + lane;
-  const bool olive = o < Po.ng;
   const int64_t ntile = ((int64_t)Po.nz + GF_ZT - 1) / GF_ZT;
   const int nh = Pi.ng;
+  for (int yb = blockIdx.y; yb < M * noc; yb += gridDim.y) {  // the grid is capped in y
+  const int nm = yb / noc;
+  const int o0 = (yb - nm * noc) * 64;
+  const int64_t nl = m2l[nm];
+  if (nl < 0 || nl >= L1) continue;
+  const int o = o0 + lane;
+  const bool olive = o < Po.ng;
   for (int64_t t = (int64_t)blockIdx.x * 4 + wave; t < ntile; t += (int64_t)gridDim.x * 4) {
     const int64_t z0 = t * GF_ZT;
     const int nzt = (int)((Po.nz - z0) < GF_ZT ? (Po.nz - z0) : GF_ZT);
@@ -635,6 +645,7 @@ __global__ __launch_bounds__(256) void k_scatter_dense_gfast(const DevPR *__rest
         }
       }
     }
+  }
   }
 }
 #undef KR_DENSE_PAIR
@@ -959,20 +970,18 @@ int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t 
         const int nocg = (int)((L.max_g + SG_OC - 1) / SG_OC);
         const int64_t nst = (L.max_z + MF_ZS - 1) / MF_ZS;
         const int64_t nblk = ((nst + 7) / 8) * 8 * L1 * nocg;
-        if (nblk > 0x7fffffffLL) {
-          set_error("scattering: too many zones for the dense moment-fastest grid");
-          return KR_ERR_UNSUPPORTED;
-        }
+        const unsigned gx = (unsigned)std::min<int64_t>(nblk, DENSE_MAX_BLOCKS_MFAST);  // both multiples of 8
         if (adjoint)
-          hipLaunchKernelGGL(k_scatter_dense_mfast<true>, dim3((unsigned)nblk), dim3(256), 0, L.stream, L.pr, W,
-                             wstride, L.sigs_t, tab, nmat, L1, G, s, d, acc, flags, nocg);
+          hipLaunchKernelGGL(k_scatter_dense_mfast<true>, dim3(gx), dim3(256), 0, L.stream, L.pr, W, wstride,
+                             L.sigs_t, tab, nmat, L1, G, s, d, acc, flags, nocg, nblk);
         else
-          hipLaunchKernelGGL(k_scatter_dense_mfast<false>, dim3((unsigned)nblk), dim3(256), 0, L.stream, L.pr, W,
-                             wstride, L.sigs_t, tab, nmat, L1, G, s, d, acc, flags, nocg);
+          hipLaunchKernelGGL(k_scatter_dense_mfast<false>, dim3(gx), dim3(256), 0, L.stream, L.pr, W, wstride,
+                             L.sigs_t, tab, nmat, L1, G, s, d, acc, flags, nocg, nblk);
       } else if (L.gfast) {
         const int64_t tiles = (L.max_z + GF_ZT - 1) / GF_ZT;
-        dim3 grid((unsigned)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1 << 20)),
-                  (unsigned)(M * ((L.max_g + 63) / 64)));
+        const int64_t gxf = std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1 << 20));
+        const int64_t gyf = (int64_t)M * ((L.max_g + 63) / 64);
+        dim3 grid((unsigned)gxf, (unsigned)std::max<int64_t>(1, std::min<int64_t>(gyf, DENSE_MAX_BLOCKS / gxf)));
         if (adjoint)
           hipLaunchKernelGGL(k_scatter_dense_gfast<true>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l,
                              M, nmat, L1, G, s, d, acc, flags);
@@ -980,7 +989,9 @@ int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t 
           hipLaunchKernelGGL(k_scatter_dense_gfast<false>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l,
                              M, nmat, L1, G, s, d, acc, flags);
       } else {
-        dim3 grid((unsigned)((L.max_z + 255) / 256), (unsigned)(M * ((L.max_g + SG_OC - 1) / SG_OC)));
+        const int64_t gxf = (L.max_z + 255) / 256;
+        const int64_t gyf = (int64_t)M * ((L.max_g + SG_OC - 1) / SG_OC);
+        dim3 grid((unsigned)gxf, (unsigned)std::max<int64_t>(1, std::min<int64_t>(gyf, DENSE_MAX_BLOCKS / gxf)));
         if (adjoint)
           hipLaunchKernelGGL(k_scatter_dense<true>, grid, dim3(256), 0, L.stream, L.pr, W, wstride, L.sigs_t, L.m2l, M,
                              nmat, L1, G, s, d, acc, flags);
