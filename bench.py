@@ -166,13 +166,16 @@ BATCH_PROBLEMS, BATCH_ITERATIONS = 48, 15
 
 def batch_configs(n=BATCH_PROBLEMS, base_seed=20261202):
     """The batched workload: n independent problems, per-problem seed
-    base + index, zones 16 +- 4 per axis, G in {8,16,32,64}, D in
-    {32,64,96,128} (GL, 8 polar angles), Legendre order 1..4, DGZ."""
+    base + index; zones per axis n0 + jitter with n0 drawn from 8..40 and a
+    per-axis jitter of -2..2 (clipped to 8..40), G in {8,16,32,64}, D in
+    {32,64,96,128} (GL, 8 polar angles), Legendre order 1..4.  The nesting is
+    left to the autotuner (run_batched)."""
     import numpy as np
     rng = np.random.default_rng(base_seed)
     out = []
     for i in range(n):
-        zones = tuple(int(v) for v in np.clip(16 + rng.integers(-4, 5, size=3), 8, 40))
+        n0 = int(rng.integers(8, 41))
+        zones = tuple(int(v) for v in np.clip(n0 + rng.integers(-2, 3, size=3), 8, 40))
         out.append(dict(zones=zones, groups=int(rng.choice([8, 16, 32, 64])),
                         quad=(8, int(rng.choice([4, 8, 12, 16]))), legendre=int(rng.integers(1, 5)), layout="DGZ",
                         dset=8, gset=1, zset=(1, 1, 1), geometry=1, seed=base_seed + i))
@@ -181,12 +184,16 @@ def batch_configs(n=BATCH_PROBLEMS, base_seed=20261202):
 
 def run_batched():
     """48 independent problems x 15 iterations: one hipGraph launch (each
-    problem's sweep shape autotuned, cached per shape) against the same
-    iterations issued problem by problem, one call at a time."""
+    problem's nesting and sweep shape picked by the autotuner) against the
+    same iterations issued problem by problem, one call at a time."""
     import numpy as np
     import kripke_amd as ka
     ctx = ka.Context(0)
     cfgs = batch_configs()
+    t0 = time.perf_counter()
+    for c in cfgs:
+        c["layout"] = ka.autotune_nesting(ctx, ka.input_vars(**c))[0]
+    nest_s = time.perf_counter() - t0
     hps = [ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs]
     t0 = time.perf_counter()
     for hp in hps:
@@ -217,11 +224,14 @@ def run_batched():
     for hp in hps:
         hp.close()
     ctx.close()
-    return {"workload": "%d problems (zones 16+-4 per axis, G in {8,16,32,64}, D in {32,64,96,128}, P1-P4, DGZ), "
-                        "%d iterations each" % (BATCH_PROBLEMS, BATCH_ITERATIONS),
+    nest = {l: sum(1 for c in cfgs if c["layout"] == l) for l in ka.LAYOUTS}
+    return {"workload": "%d problems (zones n0+-2 per axis, n0 in 8..40; G in {8,16,32,64}, D in {32,64,96,128}, "
+                        "P1-P4; nesting and sweep shape autotuned), %d iterations each" % (BATCH_PROBLEMS,
+                                                                                      BATCH_ITERATIONS),
             "unknowns_per_iteration": unknowns, "graph_ms": graph_s * 1e3, "per_problem_ms": seq_s * 1e3,
             "speedup_vs_per_problem": seq_s / graph_s, "graph_unknowns_per_s": unknowns * BATCH_ITERATIONS / graph_s,
-            "populations_equal_per_problem_runs": same, "autotune_s": tune_s}
+            "populations_equal_per_problem_runs": same, "nestings": {k: v for k, v in nest.items() if v},
+            "autotune_nesting_s": nest_s, "autotune_s": tune_s}
 
 
 def run_cpu_baseline():

@@ -214,13 +214,15 @@ int kr_problem_autotune(kr_problem *prob, int32_t *choice);
  * registered kr_problem on one context) advanced together.  kr_batch_iterate
  * runs niter source iterations of every problem -- the on-device pipeline:
  * LTimes, scattering, source(strength), LPlusTimes, SweepSolver with the
- * population -- as ONE hipGraph launch with one branch per problem, and
- * returns the populations as populations[problem*niter + iteration].  Each
- * problem's results equal those of its own iterations run one call at a time
- * (same kernels in the same order).  The graph is captured on the first call
- * and re-captured when niter changes; the problems' fields must not be
- * re-registered while the batch lives.  Blocks until the values are on the
- * host. */
+ * population -- as ONE hipGraph launch of multi-problem kernels (one launch
+ * covers the chunks or sweep units of every problem that takes the same
+ * kernel variant), and returns the populations as
+ * populations[problem*niter + iteration].  Each problem's results equal
+ * those of its own iterations run one call at a time, bit for bit.  The
+ * graph is captured on the first call and re-captured when niter changes, a
+ * problem is re-tuned, or a problem's sigs changes between group-diagonal
+ * and coupled; the problems' fields must not be re-registered while the
+ * batch lives.  Blocks until the values are on the host. */
 typedef struct kr_batch kr_batch;
 int kr_batch_create(kr_context *ctx, kr_problem *const *problems, const double *source_strength, int32_t n,
                     kr_batch **out);
@@ -284,6 +286,15 @@ int kr_host_field_device(kr_host_problem *hp, const char *name, int32_t chunk, v
 int kr_host_init_psi(kr_host_problem *hp);
 int kr_host_iterate(kr_host_problem *hp, int32_t mode, double *population);
 int kr_host_solve(kr_host_problem *hp, int32_t niter, int32_t mode, double *populations);
+/* No counterpart in the reference (its nesting is the --layout option,
+ * src/Kripke/Generate.cpp:56-70 dispatches on it).  Picks the nesting for a
+ * problem: builds it in each of the six nestings on ctx, tunes each one's
+ * sweep shape (kr_problem_autotune), and times one on-device source
+ * iteration (best of two after a warm-up); returns the fastest in *nesting
+ * and its time in *ms (may be NULL).  Cached per context for problems of
+ * the same extents, quadrature, groups, order and sets.  Nestings whose
+ * shapes the kernels do not cover are skipped. */
+int kr_host_autotune_nesting(kr_context *ctx, const kr_input_vars *in, int32_t *nesting, double *ms);
 
 #ifdef __cplusplus
 }

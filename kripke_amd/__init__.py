@@ -3,5 +3,5 @@
 The product is libkripke_amd.so (HIP kernels for gfx950 + C++ host mirror)
 behind the C-ABI in include/kripke_amd.h; this package only binds it.
 """
-from .capi import (LAYOUTS, Batch, Context, HostProblem, InputVars, KernelProblem, KripkeError, device_count,  # noqa: F401
-                   input_vars, lib)
+from .capi import (LAYOUTS, Batch, Context, HostProblem, InputVars, KernelProblem, KripkeError,  # noqa: F401
+                   autotune_nesting, device_count, input_vars, lib)

@@ -133,6 +133,7 @@ SIGNATURES = [
     ("kr_host_init_psi", ctypes.c_int, [P]),
     ("kr_host_iterate", ctypes.c_int, [P, ctypes.c_int32, DP]),
     ("kr_host_solve", ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, DP]),
+    ("kr_host_autotune_nesting", ctypes.c_int, [P, ctypes.POINTER(InputVars), ctypes.POINTER(ctypes.c_int32), DP]),
 ]
 
 _lib = None
@@ -162,6 +163,17 @@ def device_count():
     n = ctypes.c_int32()
     check(lib().kr_device_count(ctypes.byref(n)), "kr_device_count")
     return n.value
+
+
+def autotune_nesting(ctx, inp):
+    """kr_host_autotune_nesting: the fastest of the six nestings for the
+    problem `inp` on ctx (each built, sweep-tuned and timed for one source
+    iteration); returns (layout name, ms per iteration).  Cached per context
+    and shape."""
+    nest, ms = ctypes.c_int32(), ctypes.c_double()
+    check(lib().kr_host_autotune_nesting(ctx.h, ctypes.byref(inp), ctypes.byref(nest), ctypes.byref(ms)),
+          "kr_host_autotune_nesting")
+    return LAYOUTS[nest.value], ms.value
 
 
 class Context:

@@ -1,7 +1,11 @@
 // C-ABI over the host mirror (kr_host_* in include/kripke_amd.h).
 // C++ exceptions stop here and become status codes.
+#include <chrono>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "kripke_host.h"
 
@@ -61,6 +65,10 @@ int get_chunk(kr_context *ctx, Field<T> &f, int32_t chunk, double *dst, int64_t 
   for (size_t i = 0; i < cnt; ++i) dst[i] = (double)tmp[i];
   return KR_OK;
 }
+
+// Nesting choices of kr_host_autotune_nesting, per context and problem shape.
+std::mutex g_nest_mu;
+std::map<std::pair<const kr_context *, std::vector<int64_t>>, std::pair<int32_t, double>> g_nest_cache;
 
 }  // namespace
 
@@ -184,6 +192,69 @@ int kr_host_solve(kr_host_problem *hp, int32_t niter, int32_t mode, double *pops
     rc = kr_host_iterate(hp, mode, pops ? pops + it : nullptr);
     if (rc) return rc;
   }
+  return KR_OK;
+}
+
+// Times one source iteration of the problem in each of the six nestings
+// (each with its sweep shape tuned) and returns the fastest.  The choice is
+// cached per context for every problem of the same extents and sets.
+int kr_host_autotune_nesting(kr_context *ctx, const kr_input_vars *in, int32_t *nesting, double *ms) {
+  if (!ctx || !in || !nesting) {
+    kr::set_error("kr_host_autotune_nesting: null argument");
+    return KR_ERR_INVALID;
+  }
+  const std::vector<int64_t> key = {in->nx, in->ny, in->nz, in->num_directions, in->quad_num_polar,
+                                    in->quad_num_azimuthal, in->num_groups, in->legendre_order, in->num_dirsets,
+                                    in->num_groupsets, in->num_zonesets[0], in->num_zonesets[1], in->num_zonesets[2],
+                                    in->parallel_method};
+  {
+    std::lock_guard<std::mutex> lk(g_nest_mu);
+    auto it = g_nest_cache.find({ctx, key});
+    if (it != g_nest_cache.end()) {
+      *nesting = it->second.first;
+      if (ms) *ms = it->second.second;
+      return KR_OK;
+    }
+  }
+  int32_t best = -1;
+  double best_ms = 1e300;
+  std::string last_error;
+  for (int32_t lay = KR_DGZ; lay <= KR_ZGD; ++lay) {
+    kr_input_vars v = *in;
+    v.layout = lay;
+    kr_host_problem *hp = nullptr;
+    int rc = kr_host_create(ctx, &v, &hp);
+    if (rc == KR_OK) rc = kr_problem_autotune(kr_host_kernel_problem(hp), nullptr);
+    if (rc == KR_OK) rc = kr_host_init_psi(hp);
+    if (rc == KR_OK) rc = kr_host_iterate(hp, 1, nullptr);  // warm-up
+    double t = 1e300;
+    for (int rep = 0; rep < 2 && rc == KR_OK; ++rep) {
+      const auto t0 = std::chrono::steady_clock::now();
+      rc = kr_host_iterate(hp, 1, nullptr);  // ends with the population read-back
+      t = std::min(t, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    if (rc != KR_OK) last_error = kr_last_error();
+    if (hp) kr_host_destroy(hp);
+    if (rc == KR_ERR_UNSUPPORTED) continue;  // a nesting whose shapes the kernels do not cover
+    if (rc != KR_OK) {
+      kr::set_error("kr_host_autotune_nesting: " + last_error);
+      return rc;
+    }
+    if (t < best_ms) {
+      best_ms = t;
+      best = lay;
+    }
+  }
+  if (best < 0) {
+    kr::set_error("kr_host_autotune_nesting: no nesting supports this problem (" + last_error + ")");
+    return KR_ERR_UNSUPPORTED;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_nest_mu);
+    g_nest_cache[{ctx, key}] = {best, best_ms};
+  }
+  *nesting = best;
+  if (ms) *ms = best_ms;
   return KR_OK;
 }
 

@@ -1111,63 +1111,106 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
 
 // ---- batched persistent launch ----------------------------------------------
 
-// Device buffer of one population partial per pipelined sweep unit.
-static int ensure_pop_units(kr_problem *p, int64_t *n) {
-  *n = p->pipe_level_begin.empty() ? 0 : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
-  if (*n > p->pop_units_cap) {
-    if (p->d_pop_units) KR_HIP_CHECK(hipFree(p->d_pop_units));
-    p->d_pop_units = nullptr;
-    KR_HIP_CHECK(hipMalloc((void **)&p->d_pop_units, std::max<int64_t>(*n, 1) * sizeof(double)));
-    p->pop_units_cap = *n;
-  }
-  return KR_OK;
-}
+// A batch runs its problems' iterations as multi-problem launches: the
+// tables of every problem (subdomains, PR and R chunks, planes, sweep units)
+// are concatenated, with their indices shifted, and one launch covers the
+// chunks or units of all problems that take the same kernel variant.  Each
+// chunk's and each unit's result depends only on its own tables, so every
+// value equals the problem's own launches bit for bit; the population of a
+// problem adds its units' partials in its own unit order.
+constexpr int BATCH_BRANCHES = 8;
 
-// One source iteration of the on-device pipeline (the mode-1 sequence of the
-// host mirror's SteadyStateIteration) enqueued on stream st, with the
-// population written to pop_dev.  Only kernel launches: capturable.
-static int iterate_on(kr_problem *p, hipStream_t st, double strength, double *pop_dev) {
-  LaunchCtx L = launch_ctx(p);
-  L.stream = st;
-  int rc;
-  if ((rc = launch_ltimes(L, 0)) || (rc = launch_scattering(L, p->pr_gset.data(), p->pr_r.data(), 0)) ||
-      (rc = launch_source(L, strength)) || (rc = launch_lplustimes(L, 0)))
-    return rc;
-  if ((rc = launch_sweep_coef(st, p->d_sd, 0, p->nsdom))) return rc;
-  for (size_t lv = 0; lv < p->pipe_level_begin.size(); ++lv) {
-    rc = launch_sweep(st, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
-                      p->pipe_level_count[lv], p->shape_pipe, p->d_pop_units + p->pipe_level_begin[lv]);
-    if (rc) return rc;
-  }
-  const int64_t n = p->pipe_level_begin.empty() ? 0
-                                                : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
-  return launch_sum(st, p->d_pop_units, n, pop_dev);
-}
+struct BatchSweep {
+  SweepShape shape;     // the members' shape (row-scan kernel: their common RG, V, PK)
+  int64_t coef = 0;     // LDS coefficient table (doubles): the largest over the members
+  int32_t ubeg = 0, ucnt = 0;
+};
+
+struct BatchLt {
+  int M = 0, nd = 0, dfast = 0;
+  int32_t pbeg = 0, pcnt = 0;  // range of the group's reordered PR table
+  int64_t max_gz = 0;
+};
 
 struct kr_batch {
   kr_context *ctx = nullptr;
   std::vector<kr_problem *> probs;
   std::vector<double> strength;
-  std::vector<hipStream_t> streams;  // one branch per problem while capturing
-  std::vector<hipEvent_t> joins;
-  hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int32_t graph_niter = -1;
   std::vector<int64_t> versions;     // the problems' schedule versions at capture
+  std::vector<int32_t> coupled;      // per problem: its sigs couples groups or moments (at capture)
   double *d_pop = nullptr;           // [problem][iteration]
   int64_t pop_cap = 0;
+  // combined tables (one device block)
+  char *d_tab = nullptr;
+  DevSdom *d_sd = nullptr;
+  DevPR *d_pr = nullptr, *d_pr_lt = nullptr, *d_pr_lp = nullptr;
+  DevR *d_r = nullptr;
+  int32_t *d_pr_sdoms = nullptr;
+  SweepPlanes *d_planes = nullptr;
+  SweepUnit *d_units = nullptr;
+  DevScat *d_scat_prob = nullptr, *d_scat_chunk = nullptr;
+  int32_t *d_flags = nullptr;        // SIGS_FLAG_BLOCKS per problem
+  int64_t *d_pop_list = nullptr, *d_pop_begin = nullptr;
+  double *d_pop_units = nullptr;     // one population partial per combined unit
+  int32_t nsd = 0, npr = 0;
+  int64_t max_gz = 0, max_gmix = 0;
+  std::vector<BatchLt> lt, lp;
+  std::vector<std::vector<BatchSweep>> levels;
+  hipStream_t branch[BATCH_BRANCHES] = {};
+  hipEvent_t join[BATCH_BRANCHES] = {};
+  hipEvent_t fork = nullptr;
 };
+
+}  // extern "C"
+
+// Enqueues launch(stream, i) for i < n on BATCH_BRANCHES streams forked from
+// s0 and joined back (graph branches when captured): independent launches of
+// one step may overlap, so the tail of one fills with the start of another.
+template <class F>
+static int batch_parallel(kr_batch *b, hipStream_t s0, int n, F &&launch) {
+  if (n <= 1) return n == 1 ? launch(s0, 0) : KR_OK;
+  KR_HIP_CHECK(hipEventRecord(b->fork, s0));
+  const int used = std::min(n, BATCH_BRANCHES);
+  for (int k = 0; k < used; ++k) KR_HIP_CHECK(hipStreamWaitEvent(b->branch[k], b->fork, 0));
+  for (int i = 0; i < n; ++i) {
+    int rc = launch(b->branch[i % BATCH_BRANCHES], i);
+    if (rc) return rc;
+  }
+  for (int k = 0; k < used; ++k) {
+    KR_HIP_CHECK(hipEventRecord(b->join[k], b->branch[k]));
+    KR_HIP_CHECK(hipStreamWaitEvent(s0, b->join[k], 0));
+  }
+  return KR_OK;
+}
+
+extern "C" {
+
+static void batch_free_tables(kr_batch *b) {
+  if (b->d_tab) hipFree(b->d_tab);
+  if (b->d_pop_units) hipFree(b->d_pop_units);
+  b->d_tab = nullptr;
+  b->d_pop_units = nullptr;
+  b->lt.clear();
+  b->lp.clear();
+  b->levels.clear();
+}
 
 int kr_batch_destroy(kr_batch *b) {
   if (!b) return KR_OK;
   if (b->ctx) hipStreamSynchronize(b->ctx->stream);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
-  for (hipStream_t s : b->streams) hipStreamDestroy(s);
-  for (hipEvent_t e : b->joins) hipEventDestroy(e);
-  if (b->fork) hipEventDestroy(b->fork);
   if (b->d_pop) hipFree(b->d_pop);
+  if (b->d_flags) hipFree(b->d_flags);
+  for (int k = 0; k < BATCH_BRANCHES; ++k) {
+    if (b->branch[k]) hipStreamDestroy(b->branch[k]);
+    if (b->join[k]) hipEventDestroy(b->join[k]);
+  }
+  if (b->fork) hipEventDestroy(b->fork);
+  batch_free_tables(b);
   delete b;
   return KR_OK;
 }
@@ -1192,26 +1235,310 @@ int kr_batch_create(kr_context *ctx, kr_problem *const *problems, const double *
     b->probs.push_back(problems[i]);
     b->strength.push_back(source_strength[i]);
   }
-  b->streams.resize(n, nullptr);
-  b->joins.resize(n, nullptr);
-  for (int32_t i = 0; i < n; ++i) {
-    if (hipStreamCreateWithFlags(&b->streams[i], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->joins[i], hipEventDisableTiming) != hipSuccess) {
-      kr_batch_destroy(b);
-      return fail(KR_ERR_HIP, "kr_batch_create: stream/event creation failed");
-    }
-  }
-  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+  if (hipMalloc((void **)&b->d_flags, sizeof(int32_t) * SIGS_FLAG_BLOCKS * n) != hipSuccess) {
+    b->d_flags = nullptr;
     kr_batch_destroy(b);
-    return fail(KR_ERR_HIP, "kr_batch_create: event creation failed");
+    return fail(KR_ERR_NOMEM, "kr_batch_create: structure flags");
+  }
+  bool ok = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; ok && k < BATCH_BRANCHES; ++k)
+    ok = hipStreamCreateWithFlags(&b->branch[k], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    kr_batch_destroy(b);
+    return fail(KR_ERR_HIP, "kr_batch_create: stream/event creation failed");
   }
   *out = b;
   return KR_OK;
 }
 
-// Builds the graph: a fork from the context stream, one branch per problem
-// holding niter iterations, a join back.
-static int batch_capture(kr_batch *b, int32_t niter) {
+}  // extern "C"
+
+// Host image of the combined tables: each part 256-byte aligned in one block.
+struct TabImage {
+  std::vector<char> bytes;
+  template <class T>
+  size_t add(const std::vector<T> &v) {
+    const size_t off = (bytes.size() + 255) & ~(size_t)255;
+    bytes.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+extern "C" {
+
+// Classifies every problem's sigs into the batch's flags and returns, per
+// problem, whether it couples groups or moments.
+static int batch_coupling(kr_batch *b, std::vector<int32_t> &coupled) {
+  const size_t n = b->probs.size();
+  hipStream_t st = b->ctx->stream;
+  int rc = launch_sigs_structure_multi(st, b->d_scat_prob, (int)n);
+  if (rc) return rc;
+  std::vector<int32_t> f;
+  if ((rc = download(b->ctx, f, b->d_flags, n * SIGS_FLAG_BLOCKS))) return rc;
+  coupled.assign(n, 0);
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < SIGS_FLAG_BLOCKS; ++k) coupled[i] |= f[i * SIGS_FLAG_BLOCKS + k] != 0;
+  return KR_OK;
+}
+
+// Builds the combined tables and the launch lists from the problems' current
+// device tables and pipelined sweep schedules.
+static int batch_build(kr_batch *b) {
+  batch_free_tables(b);
+  kr_context *ctx = b->ctx;
+  const int np = (int)b->probs.size();
+  std::vector<DevSdom> sd;
+  std::vector<DevPR> pr;
+  std::vector<DevR> rr;
+  std::vector<int32_t> prs, pr_prob;
+  std::vector<SweepPlanes> planes;
+  std::vector<DevScat> scat_prob, scat_chunk;
+  std::vector<int32_t> sd_off(np), pr_off(np);
+  // per problem: its pipelined units (combined sdom ids, PK in `reserved`)
+  std::vector<std::vector<SweepUnit>> units(np);
+  int rc;
+  b->max_gz = b->max_gmix = 0;
+  for (int i = 0; i < np; ++i) {
+    kr_problem *p = b->probs[i];
+    sd_off[i] = (int32_t)sd.size();
+    pr_off[i] = (int32_t)pr.size();
+    const int32_t r_off = (int32_t)rr.size(), s_off = (int32_t)prs.size();
+    std::vector<DevR> hr;
+    std::vector<int32_t> hprs;
+    std::vector<SweepPlanes> hpl;
+    int64_t npl = 0;
+    for (const DevPR &P : p->h_pr) npl = std::max<int64_t>(npl, (int64_t)P.sdom_begin + P.sdom_count);
+    const int64_t nu = p->pipe_level_begin.empty() ? 0 : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
+    if ((rc = download(ctx, hr, p->d_r, p->nr)) || (rc = download(ctx, hprs, p->d_pr_sdoms, npl)) ||
+        (rc = download(ctx, hpl, p->d_planes_pipe, p->nsdom)) || (rc = download(ctx, units[i], p->d_units_pipe, nu)))
+      return rc;
+    for (DevSdom S : p->h_sd) {
+      S.pr += pr_off[i];
+      S.r += r_off;
+      sd.push_back(S);
+    }
+    for (DevPR P : p->h_pr) {
+      P.r += r_off;
+      P.sdom_begin += s_off;
+      pr.push_back(P);
+      pr_prob.push_back(i);
+      b->max_gz = std::max<int64_t>(b->max_gz, (int64_t)P.ng * P.nz);
+    }
+    for (const DevR &R : hr) {
+      rr.push_back(R);
+    }
+    for (size_t k = 0; k < p->h_pr.size(); ++k)
+      b->max_gmix = std::max<int64_t>(b->max_gmix, (int64_t)p->h_pr[k].ng * hr[p->h_pr[k].r].num_mix);
+    for (int32_t s : hprs) prs.push_back(s + sd_off[i]);
+    planes.insert(planes.end(), hpl.begin(), hpl.end());
+    DevScat Q{};
+    Q.sigs = p->sigs;
+    for (int k = 0; k < 4; ++k) Q.str[k] = p->sigs_str[k];
+    Q.nmat = p->nmat;
+    Q.L1 = p->L1;
+    Q.G = p->G;
+    Q.strength = b->strength[i];
+    Q.M = p->M;
+    Q.problem = i;
+    scat_prob.push_back(Q);  // flags pointer set after the upload
+    for (SweepUnit &u : units[i]) u.sdom += sd_off[i];
+  }
+  b->nsd = (int32_t)sd.size();
+  b->npr = (int32_t)pr.size();
+  for (int32_t k = 0; k < b->npr; ++k) scat_chunk.push_back(scat_prob[pr_prob[k]]);
+
+  // LTimes / LPlusTimes groups: chunks of equal kernel variant, reordered
+  // so each group is one contiguous range
+  auto group = [&](bool lplus, std::vector<DevPR> &order, std::vector<BatchLt> &out) {
+    std::map<std::pair<int64_t, int>, std::vector<int32_t>> g;  // (variant, dfast) -> chunks
+    std::vector<int64_t> key(np);
+    std::vector<int> dfast(np);
+    for (int i = 0; i < np; ++i) {
+      LaunchCtx L = launch_ctx(b->probs[i]);
+      key[i] = ltimes_variant(L, lplus);
+      dfast[i] = L.dfast;
+    }
+    for (int32_t k = 0; k < b->npr; ++k) g[{key[pr_prob[k]], dfast[pr_prob[k]]}].push_back(k);
+    for (auto &kv : g) {
+      BatchLt G;
+      const kr_problem *p0 = b->probs[pr_prob[kv.second[0]]];
+      G.M = p0->M;
+      G.nd = p0->h_sd[0].nd;
+      G.dfast = kv.first.second;
+      G.pbeg = (int32_t)order.size();
+      for (int32_t k : kv.second) {
+        order.push_back(pr[k]);
+        G.max_gz = std::max<int64_t>(G.max_gz, (int64_t)pr[k].ng * pr[k].nz);
+      }
+      G.pcnt = (int32_t)kv.second.size();
+      out.push_back(G);
+    }
+  };
+  std::vector<DevPR> pr_lt, pr_lp;
+  group(false, pr_lt, b->lt);
+  group(true, pr_lp, b->lp);
+
+  // sweep launches per dependency level: the row-scan kernel takes units of
+  // any extents with the same row groups (RG, V) and pair order; other
+  // shapes share a launch only when equal
+  std::vector<SweepUnit> cu;                // combined units, launch by launch
+  std::vector<std::vector<int64_t>> pos(np);  // combined position of each problem's unit
+  for (int i = 0; i < np; ++i) pos[i].assign(units[i].size(), -1);
+  size_t nlev = 0;
+  for (kr_problem *p : b->probs) nlev = std::max(nlev, p->pipe_level_begin.size());
+  for (size_t lv = 0; lv < nlev; ++lv) {
+    std::vector<std::vector<int>> groups;
+    std::vector<SweepShape> gshape;
+    for (int i = 0; i < np; ++i) {
+      const kr_problem *p = b->probs[i];
+      if (lv >= p->pipe_level_begin.size()) continue;
+      const SweepShape &s = p->shape_pipe;
+      size_t gi = 0;
+      for (; gi < groups.size(); ++gi) {
+        const SweepShape &t = gshape[gi];
+        const bool same = s.kind == 1 ? (t.kind == 1 && t.RG == s.RG && t.V == s.V && t.PK == s.PK &&
+                                         t.inner_is_g == s.inner_is_g)
+                                      : memcmp(&s, &t, sizeof(SweepShape)) == 0;
+        if (same) break;
+      }
+      if (gi == groups.size()) {
+        groups.emplace_back();
+        gshape.push_back(s);
+      }
+      groups[gi].push_back(i);
+    }
+    std::vector<BatchSweep> launches;
+    auto emit = [&](const std::vector<int> &members, SweepShape sh) {
+      BatchSweep B;
+      int64_t coef = 0;
+      for (int i : members) {
+        const DevSdom &S = b->probs[i]->h_sd[0];
+        coef = std::max<int64_t>(coef, (int64_t)S.nd * (S.ni + S.nj + S.nk + 1));
+      }
+      const int nt = sh.kind == 1 ? sh.RG * sh.PK : sh.Q * sh.PJ * sh.PK;
+      const size_t lds = ((sh.kind == 1 ? 2 * (size_t)sh.V : 4) * nt + (size_t)coef) * sizeof(double);
+      if (lds > ctx->lds_max || nt > 1024) return false;
+      B.shape = sh;
+      B.coef = coef;
+      B.ubeg = (int32_t)cu.size();
+      for (int i : members) {
+        const kr_problem *p = b->probs[i];
+        const int32_t ub = p->pipe_level_begin[lv], uc = p->pipe_level_count[lv];
+        for (int32_t u = ub; u < ub + uc; ++u) {
+          pos[i][u] = (int64_t)cu.size();
+          cu.push_back(units[i][u]);
+        }
+      }
+      B.ucnt = (int32_t)cu.size() - B.ubeg;
+      launches.push_back(B);
+      return true;
+    };
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      if (emit(groups[gi], gshape[gi])) continue;
+      for (int i : groups[gi])  // too large together: each problem on its own
+        if (!emit({i}, b->probs[i]->shape_pipe)) return fail(KR_ERR_UNSUPPORTED, "batch: sweep shape does not fit");
+    }
+    b->levels.push_back(launches);
+  }
+  std::vector<int64_t> plist, pbegin{0};
+  for (int i = 0; i < np; ++i) {
+    for (int64_t q : pos[i]) {
+      if (q < 0) return fail(KR_ERR_INVALID, "batch: a sweep unit outside every level");
+      plist.push_back(q);
+    }
+    pbegin.push_back((int64_t)plist.size());
+  }
+
+  TabImage im;
+  const size_t o_sd = im.add(sd), o_pr = im.add(pr), o_lt = im.add(pr_lt), o_lp = im.add(pr_lp), o_r = im.add(rr),
+               o_prs = im.add(prs), o_pl = im.add(planes), o_u = im.add(cu), o_sp = im.add(scat_prob),
+               o_sc = im.add(scat_chunk), o_l = im.add(plist), o_b = im.add(pbegin);
+  KR_HIP_CHECK(hipMalloc((void **)&b->d_tab, std::max<size_t>(im.bytes.size(), 256)));
+  KR_HIP_CHECK(hipMalloc((void **)&b->d_pop_units, sizeof(double) * std::max<size_t>(cu.size(), 1)));
+  char *base = b->d_tab;
+  b->d_sd = (DevSdom *)(base + o_sd);
+  b->d_pr = (DevPR *)(base + o_pr);
+  b->d_pr_lt = (DevPR *)(base + o_lt);
+  b->d_pr_lp = (DevPR *)(base + o_lp);
+  b->d_r = (DevR *)(base + o_r);
+  b->d_pr_sdoms = (int32_t *)(base + o_prs);
+  b->d_planes = (SweepPlanes *)(base + o_pl);
+  b->d_units = (SweepUnit *)(base + o_u);
+  b->d_scat_prob = (DevScat *)(base + o_sp);
+  b->d_scat_chunk = (DevScat *)(base + o_sc);
+  b->d_pop_list = (int64_t *)(base + o_l);
+  b->d_pop_begin = (int64_t *)(base + o_b);
+  // flags: each problem's share of the batch's flag array
+  for (int i = 0; i < np; ++i) {
+    DevScat *q = (DevScat *)(im.bytes.data() + o_sp) + i;
+    q->flags = b->d_flags + (int64_t)i * SIGS_FLAG_BLOCKS;
+  }
+  for (int32_t k = 0; k < b->npr; ++k) {
+    DevScat *q = (DevScat *)(im.bytes.data() + o_sc) + k;
+    q->flags = b->d_flags + (int64_t)pr_prob[k] * SIGS_FLAG_BLOCKS;
+  }
+  KR_HIP_CHECK(hipMemcpyAsync(b->d_tab, im.bytes.data(), im.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  return KR_OK;
+}
+
+// One source iteration of every problem (the mode-1 sequence of the host
+// mirror's SteadyStateIteration), population into out[problem*stride].
+// Only kernel launches: capturable.
+static int batch_iteration(kr_batch *b, hipStream_t st, double *out, int64_t stride) {
+  int rc;
+  auto ctx_of = [&](const BatchLt &G, DevPR *tab) {
+    LaunchCtx L{};
+    L.stream = st;
+    L.sd = b->d_sd;
+    L.pr = tab + G.pbeg;
+    L.r = b->d_r;
+    L.pr_sdoms = b->d_pr_sdoms;
+    L.num_pr = G.pcnt;
+    L.num_moments = G.M;
+    L.nd = G.nd;
+    L.dfast = G.dfast;
+    L.lds_max = b->ctx->lds_max;
+    L.max_gz = G.max_gz;
+    return L;
+  };
+  if ((rc = batch_parallel(b, st, (int)b->lt.size(), [&](hipStream_t s, int i) {
+         LaunchCtx L = ctx_of(b->lt[i], b->d_pr_lt);
+         L.stream = s;
+         return launch_ltimes(L, 0);
+       })))
+    return rc;
+  if ((rc = launch_scatter_diag_multi(st, b->d_pr, b->d_r, b->d_scat_chunk, b->npr, b->max_gz))) return rc;
+  for (size_t i = 0; i < b->probs.size(); ++i) {
+    if (!b->coupled[i]) continue;
+    LaunchCtx L = launch_ctx(b->probs[i]);
+    L.stream = st;
+    if ((rc = launch_scattering(L, b->probs[i]->pr_gset.data(), b->probs[i]->pr_r.data(), 0))) return rc;
+  }
+  if ((rc = launch_source_multi(st, b->d_pr, b->d_r, b->d_scat_chunk, b->npr, b->max_gmix))) return rc;
+  if ((rc = batch_parallel(b, st, (int)b->lp.size(), [&](hipStream_t s, int i) {
+         LaunchCtx L = ctx_of(b->lp[i], b->d_pr_lp);
+         L.stream = s;
+         return launch_lplustimes(L, 0);
+       })))
+    return rc;
+  if ((rc = launch_sweep_coef(st, b->d_sd, 0, b->nsd))) return rc;
+  for (const auto &lv : b->levels)
+    if ((rc = batch_parallel(b, st, (int)lv.size(), [&](hipStream_t s, int i) {
+           const BatchSweep &B = lv[i];
+           if (B.shape.kind == 1)
+             return launch_sweep_rows(s, b->d_sd, b->d_planes, b->d_units + B.ubeg, B.ucnt, B.shape, B.coef,
+                                      b->d_pop_units + B.ubeg, false);
+           return launch_sweep(s, b->d_sd, b->d_planes, b->d_units + B.ubeg, B.ucnt, B.shape, b->d_pop_units + B.ubeg);
+         })))
+      return rc;
+  return launch_pop_sum_multi(st, b->d_pop_units, b->d_pop_list, b->d_pop_begin, (int)b->probs.size(), out, stride);
+}
+
+// Builds the tables and the graph of niter iterations.
+static int batch_capture(kr_batch *b, int32_t niter, const std::vector<int32_t> &coupled) {
   if (b->exec) {
     KR_HIP_CHECK(hipGraphExecDestroy(b->exec));
     b->exec = nullptr;
@@ -1227,27 +1554,12 @@ static int batch_capture(kr_batch *b, int32_t niter) {
     KR_HIP_CHECK(hipMalloc((void **)&b->d_pop, need * sizeof(double)));
     b->pop_cap = need;
   }
-  for (kr_problem *p : b->probs) {
-    int64_t nu;
-    int rc = ensure_pop_units(p, &nu);
-    if (rc) return rc;
-  }
+  b->coupled = coupled;
   hipStream_t s0 = b->ctx->stream;
   KR_HIP_CHECK(hipStreamSynchronize(s0));
   KR_HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
   int rc = KR_OK;
-  if (hipEventRecord(b->fork, s0) != hipSuccess) rc = fail(KR_ERR_HIP, "batch capture: fork");
-  for (size_t i = 0; !rc && i < b->probs.size(); ++i) {
-    if (hipStreamWaitEvent(b->streams[i], b->fork, 0) != hipSuccess) {
-      rc = fail(KR_ERR_HIP, "batch capture: branch");
-      break;
-    }
-    for (int32_t it = 0; !rc && it < niter; ++it)
-      rc = iterate_on(b->probs[i], b->streams[i], b->strength[i], b->d_pop + (int64_t)i * niter + it);
-    if (!rc && (hipEventRecord(b->joins[i], b->streams[i]) != hipSuccess ||
-                hipStreamWaitEvent(s0, b->joins[i], 0) != hipSuccess))
-      rc = fail(KR_ERR_HIP, "batch capture: join");
-  }
+  for (int32_t it = 0; !rc && it < niter; ++it) rc = batch_iteration(b, s0, b->d_pop + it, niter);
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s0, &g);
   if (rc) {
@@ -1258,19 +1570,25 @@ static int batch_capture(kr_batch *b, int32_t niter) {
   b->graph = g;
   KR_HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   b->graph_niter = niter;
-  b->versions.clear();
-  for (kr_problem *p : b->probs) b->versions.push_back(p->sched_version);
   return KR_OK;
 }
 
 int kr_batch_iterate(kr_batch *b, int32_t niter, double *populations) {
   if (!b || niter <= 0 || !populations) return fail(KR_ERR_INVALID, "kr_batch_iterate: bad argument");
-  bool stale = niter != b->graph_niter;
-  for (size_t i = 0; !stale && i < b->probs.size(); ++i) stale = b->versions[i] != b->probs[i]->sched_version;
-  if (stale) {  // first call, another iteration count, or a problem re-tuned since the capture
-    int rc = batch_capture(b, niter);
-    if (rc) return rc;
+  int rc;
+  bool rebuild = b->versions.size() != b->probs.size();
+  for (size_t i = 0; !rebuild && i < b->probs.size(); ++i) rebuild = b->versions[i] != b->probs[i]->sched_version;
+  if (rebuild) {  // first call, or a problem re-tuned since the tables were built
+    b->versions.clear();
+    if ((rc = batch_build(b))) return rc;
+    for (kr_problem *p : b->probs) b->versions.push_back(p->sched_version);
+    b->graph_niter = -1;
   }
+  // sigs as it is now decides which problems need the dense scattering
+  std::vector<int32_t> coupled;
+  if ((rc = batch_coupling(b, coupled))) return rc;
+  if (niter != b->graph_niter || coupled != b->coupled)
+    if ((rc = batch_capture(b, niter, coupled))) return rc;
   hipStream_t s0 = b->ctx->stream;
   KR_HIP_CHECK(hipGraphLaunch(b->exec, s0));
   KR_HIP_CHECK(hipMemcpyAsync(populations, b->d_pop, sizeof(double) * b->probs.size() * niter,

@@ -31,6 +31,32 @@ struct LaunchCtx {
 
 constexpr int SIGS_FLAG_BLOCKS = 64;
 
+// Scattering data of one problem, one entry per problem or per PR chunk in
+// the batched (multi-problem) launches.
+struct DevScat {
+  const double *sigs;    // data/sigs
+  int64_t str[4];        // mat, n, g, gp strides
+  int64_t nmat, L1, G;
+  int32_t *flags;        // the problem's SIGS_FLAG_BLOCKS structure flags
+  double strength;       // source strength
+  int32_t M;             // moments
+  int32_t problem;
+};
+int launch_sigs_structure_multi(hipStream_t st, const DevScat *probs, int nprob);
+// the diagonal scattering (overwrite) of every chunk whose problem's sigs is uncoupled
+int launch_scatter_diag_multi(hipStream_t st, const DevPR *pr, const DevR *r, const DevScat *chunks, int nchunks,
+                              int64_t max_gz);
+int launch_source_multi(hipStream_t st, const DevPR *pr, const DevR *r, const DevScat *chunks, int nchunks,
+                        int64_t max_gmix);
+// out[p*out_stride] = sum of partial[list[begin[p] .. begin[p+1])] (k_pop_final's order)
+int launch_pop_sum_multi(hipStream_t st, const double *partial, const int64_t *list, const int64_t *begin, int nprob,
+                         double *out, int64_t out_stride);
+
+// Kernel variant LTimes (lplus: LPlusTimes) takes for L: launches whose
+// contexts give the same value run the same kernel, and every PR chunk's
+// result depends only on its own tables, so chunks of several problems with
+// equal variants can share one launch (tables combined, nd of any of them).
+int64_t ltimes_variant(const LaunchCtx &L, bool lplus);
 int launch_ltimes(const LaunchCtx &L, int accumulate);
 int launch_lplustimes(const LaunchCtx &L, int accumulate);
 // doubles of LaunchCtx::sigs_t for the dense scattering path
@@ -60,10 +86,13 @@ int launch_copy(hipStream_t st, double *d, const double *s, int64_t n);
 // and units of the reversed dependency order).
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                  int32_t num_units, const SweepShape &shape, double *pop = nullptr, bool adjoint = false);
-// Row-scan sweep of the zone-fastest nestings (kr_sweep_rows.hip).
+// Row-scan sweep of the zone-fastest nestings (kr_sweep_rows.hip): blocks of
+// shape.RG x shape.PK threads.  Units may belong to subdomains of different
+// extents (each a multiple of shape.V along i and of shape.PK along k, with
+// at most shape.RG*shape.V zones along i); coef_doubles: the largest
+// nd*(ni+nj+nk+1) over their subdomains (the LDS coefficient table).
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
-                      double *pop, bool adjoint);
+                      int32_t num_units, const SweepShape &shape, int64_t coef_doubles, double *pop, bool adjoint);
 // Lets `kernel` take `bytes` of dynamic LDS (needed above 64 KiB).
 int allow_lds(const void *kernel, size_t bytes);
 // Sum of n doubles in a fixed order (one workgroup): out[0] = sum(partial[0..n)).

@@ -763,6 +763,89 @@ __global__ __launch_bounds__(256) void k_source(const DevPR *__restrict__ prs, c
 }
 
 // ---------------------------------------------------------------------------
+// Multi-problem forms for the batched launch: one launch covers the PR chunks
+// of many problems, each chunk carrying its problem's scattering data.
+// ---------------------------------------------------------------------------
+// k_sigs_structure for every problem of a batch: block (b, p) classifies
+// share b of problem p's sigs into its flags[b].
+__global__ __launch_bounds__(256) void k_sigs_structure_multi(const DevScat *__restrict__ probs) {
+  const DevScat &Q = probs[blockIdx.y];
+  __shared__ int any;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  const int64_t n = Q.nmat * Q.L1 * Q.G * Q.G;
+  int mine = 0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t gp_ = e % Q.G, g = (e / Q.G) % Q.G, nl = (e / (Q.G * Q.G)) % Q.L1, m = e / (Q.G * Q.G * Q.L1);
+    if ((nl > 0 || g != gp_) && Q.sigs[m * Q.str[0] + nl * Q.str[1] + g * Q.str[2] + gp_ * Q.str[3]] != 0.0) mine = 1;
+  }
+  if (mine) atomicOr(&any, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) Q.flags[blockIdx.x] = any;
+}
+
+// k_scatter_diag over the chunks of many problems (overwrite form); chunks
+// of problems whose sigs is coupled return (their problem's own dense
+// launches follow).
+__global__ __launch_bounds__(256) void k_scatter_diag_multi(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
+                                                            const DevScat *__restrict__ cs) {
+  const DevScat &Q = cs[blockIdx.y];
+  if (sigs_coupled(Q.flags)) return;
+  const DevPR &P = prs[blockIdx.y];
+  const DevR &R = rs[P.r];
+  const SigsStrides ss{Q.str[0], Q.str[1], Q.str[2], Q.str[3]};
+  const int64_t n = (int64_t)P.ng * P.nz;
+  const bool zfast = P.mz <= P.mg;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    int32_t g, z;
+    gz_of(e, P.ng, P.nz, zfast, g, z);
+    const int64_t gg = P.glo + g;
+    const double sz = sigs_zone(Q.sigs, ss, R, z, 0, gg, gg);
+    const int64_t po = g * P.mg + z * P.mz;
+    P.phi_out[po] = 0.0 + sz * P.phi[po];
+    for (int nm = 1; nm < Q.M; ++nm) P.phi_out[po + nm * P.ma] = 0.0 + 0.0;
+  }
+}
+
+// k_source over the chunks of many problems, each with its strength.
+__global__ __launch_bounds__(256) void k_source_multi(const DevPR *__restrict__ prs, const DevR *__restrict__ rs,
+                                                      const DevScat *__restrict__ cs) {
+  const DevPR &P = prs[blockIdx.y];
+  const DevR &R = rs[P.r];
+  const double strength = cs[blockIdx.y].strength;
+  const int64_t n = (int64_t)P.ng * R.num_mix;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t mix = e % R.num_mix;
+    const int32_t g = (int32_t)(e / R.num_mix);
+    if (R.mix_mat[mix] == 0) {
+      const int64_t z = R.mix_zone[mix];
+      double *q = P.phi_out + g * P.mg + z * P.mz;
+      *q = *q + strength * R.mix_frac[mix];
+    }
+  }
+}
+
+// Per problem p (block p): the sum of its sweep-unit population partials,
+// listed in its own unit order, with k_pop_final's pattern, so the value is
+// the same as that problem's own fused population.
+__global__ __launch_bounds__(256) void k_pop_sum_multi(const double *__restrict__ partial,
+                                                       const int64_t *__restrict__ list,
+                                                       const int64_t *__restrict__ begin, double *__restrict__ out,
+                                                       int64_t out_stride) {
+  __shared__ double red[256];
+  const int64_t b0 = begin[blockIdx.x], n = begin[blockIdx.x + 1] - b0;
+  double acc = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += blockDim.x) acc += partial[list[b0 + e]];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(int64_t)blockIdx.x * out_stride] = red[0];
+}
+
+// ---------------------------------------------------------------------------
 // Population: src/Kripke/Kernel/Population.cpp:49-63.  Term order per
 // element is (w*psi)*volume; the sum is a fixed two-level tree, so the
 // result does not depend on scheduling.
@@ -914,6 +997,11 @@ static int ltimes_shape(const LaunchCtx &L, int mode, const char *what, bool lpl
   return KR_OK;
 }
 
+int64_t ltimes_variant(const LaunchCtx &L, bool lplus) {
+  if (L.dfast && dfast_fits(L, L.num_moments)) return ((int64_t)L.num_moments << 32) | (1LL << 31) | dfast_stride(L.nd, L.num_moments);
+  return ((int64_t)L.num_moments << 32) | batch_of(L.nd, lplus ? 4 : 16);
+}
+
 int launch_ltimes(const LaunchCtx &L, int accumulate) { return ltimes_shape(L, accumulate ? 1 : 0, "LTimes", false); }
 int launch_lplustimes(const LaunchCtx &L, int accumulate) {
   return ltimes_shape(L, accumulate ? 1 : 0, "LPlusTimes", true);
@@ -1020,6 +1108,37 @@ int launch_scattering_zonal(const LaunchCtx &L, const double *const *sz_dev, int
   else
     hipLaunchKernelGGL(k_scatter_zonal<false>, grid, dim3(256), 0, L.stream, L.pr, sz_dev, s_g, s_z, L.num_moments,
                        accumulate);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_sigs_structure_multi(hipStream_t st, const DevScat *probs, int nprob) {
+  if (nprob <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_sigs_structure_multi, dim3(SIGS_FLAG_BLOCKS, nprob), dim3(256), 0, st, probs);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_scatter_diag_multi(hipStream_t st, const DevPR *pr, const DevR *r, const DevScat *chunks, int nchunks,
+                              int64_t max_gz) {
+  if (nchunks <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_scatter_diag_multi, dim3(grid_for(max_gz), nchunks), dim3(256), 0, st, pr, r, chunks);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_source_multi(hipStream_t st, const DevPR *pr, const DevR *r, const DevScat *chunks, int nchunks,
+                        int64_t max_gmix) {
+  if (nchunks <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_source_multi, dim3(grid_for(max_gmix), nchunks), dim3(256), 0, st, pr, r, chunks);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_pop_sum_multi(hipStream_t st, const double *partial, const int64_t *list, const int64_t *begin, int nprob,
+                         double *out, int64_t out_stride) {
+  if (nprob <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_pop_sum_multi, dim3(nprob), dim3(256), 0, st, partial, list, begin, out, out_stride);
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }

@@ -505,8 +505,8 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
     return KR_ERR_INVALID;
   }
   if (shape.kind == 1)
-    return launch_sweep_rows(st, sd, planes, units, num_units, shape, shape.nd, shape.ni, shape.nj, shape.nk, pop,
-                             adjoint);
+    return launch_sweep_rows(st, sd, planes, units, num_units, shape,
+                             (int64_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1), pop, adjoint);
   const int nt = shape.Q * shape.PJ * shape.PK;
   const size_t lds = (4 * (size_t)nt + (size_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1)) * sizeof(double);
   if (lds > 65536) {

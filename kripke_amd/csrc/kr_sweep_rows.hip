@@ -551,12 +551,11 @@ __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__
 }
 
 int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
-                      int32_t num_units, const SweepShape &shape, int32_t nd, int32_t ni, int32_t nj, int32_t nk,
-                      double *pop, bool adjoint) {
+                      int32_t num_units, const SweepShape &shape, int64_t coef_doubles, double *pop, bool adjoint) {
   if (num_units <= 0) return KR_OK;
   const int W = shape.RG, V = shape.V;
   const int nt = W * shape.PK;
-  const size_t lds = (2 * (size_t)nt * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
+  const size_t lds = (2 * (size_t)nt * V + (size_t)coef_doubles) * sizeof(double);
   switch (W * 8 + V) {
 #define X(w, v)                                                                                              \
   case w * 8 + v:                                                                                            \

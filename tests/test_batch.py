@@ -53,6 +53,60 @@ def test_batch_matches_single_problem_runs(ctx):
         hp.close()
 
 
+def zone_fastest_problems(n, seed0=300):
+    """DGZ and GDZ problems whose extents differ inside one row-scan launch:
+    odd and even ni (zones per lane 1 or 2, row groups of 8 to 32 lanes) and
+    k extents 5 to 13 (k-rows per block), one or two zone sets along x."""
+    rng = np.random.default_rng(seed0)
+    out = []
+    for i in range(n):
+        zones = (int(rng.integers(5, 14)) * (2 if i % 3 == 0 else 1), int(rng.integers(6, 12)),
+                 int(rng.integers(5, 14)))
+        out.append(dict(zones=zones, groups=int(rng.choice([4, 8])), quad=(4, int(rng.choice([2, 4]))),
+                        legendre=int(rng.integers(1, 4)), layout="DGZ" if i % 2 == 0 else "GDZ", dset=8, gset=1,
+                        zset=(2, 1, 1) if i % 3 == 0 else (1, 1, 1), geometry=1, seed=seed0 + i))
+    return out
+
+
+def test_batch_mixed_extents_and_coupled_sigs(ctx):
+    """Multi-problem launches over subdomains of different extents and the
+    dense scattering of problems whose sigs couples groups: populations and
+    phi equal each problem's own runs bit for bit, also after a problem's sigs
+    changes between calls (a re-capture)."""
+    cfgs = zone_fastest_problems(9)
+    rng = np.random.default_rng(5)
+    coupled = {}
+    for i in (0, 4):
+        hp = ka.HostProblem(ka.input_vars(**cfgs[i]), ctx)
+        coupled[i] = rng.uniform(0.0, 0.05, hp.get("data/sigs").size)
+        hp.close()
+
+    def run(hps, step):
+        hps[0].set("data/sigs", 0, coupled[0])
+        a = step(3)
+        hps[4].set("data/sigs", 0, coupled[4])
+        return np.concatenate([a, step(2)], axis=1)
+
+    ref = [ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs]
+    for hp in ref:
+        hp.init_psi()
+    want = run(ref, lambda k: np.array([[hp.iterate(1) for _ in range(k)] for hp in ref]))
+    hps = [ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs]
+    for hp in hps:
+        hp.init_psi()
+    b = ka.Batch(ctx, hps)
+    got = run(hps, b.iterate)
+    for i in range(len(cfgs)):
+        assert np.array_equal(got[i], want[i]), "problem %d (%s): %s vs %s" % (i, cfgs[i], got[i], want[i])
+        for k in range(hps[i].num_pr):
+            assert np.array_equal(hps[i].get("phi", k), ref[i].get("phi", k)), "problem %d chunk %d" % (i, k)
+        for k in range(hps[i].num_sdoms):
+            assert np.array_equal(hps[i].get("psi", k), ref[i].get("psi", k)), "problem %d sdom %d" % (i, k)
+    b.close()
+    for hp in hps + ref:
+        hp.close()
+
+
 def test_batch_rejects_bad_input(ctx):
     hp = ka.HostProblem(ka.input_vars(**mixed_problems(1)[0]), ctx)
     with pytest.raises(ka.KripkeError):
@@ -92,3 +146,23 @@ def test_autotune_keeps_results(ctx, layout):
     bt.close()
     for hp in (a, b, c):
         hp.close()
+
+
+def test_autotune_nesting(ctx):
+    """The nesting autotuner returns one of the six nestings, caches it per
+    shape, and a problem built in that nesting iterates like the DGZ one
+    (populations within the row-scan re-association)."""
+    cfg = dict(zones=(10, 9, 8), groups=8, quad=(4, 2), legendre=2, layout="DGZ", dset=8, gset=1, zset=(1, 1, 1),
+               geometry=1, seed=11)
+    lay, ms = ka.autotune_nesting(ctx, ka.input_vars(**cfg))
+    assert lay in ka.LAYOUTS and ms > 0
+    assert ka.autotune_nesting(ctx, ka.input_vars(**dict(cfg, seed=12))) == (lay, ms)  # cached per shape
+    a = ka.HostProblem(ka.input_vars(**cfg), ctx)
+    b = ka.HostProblem(ka.input_vars(**dict(cfg, layout=lay)), ctx)
+    for hp in (a, b):
+        hp.init_psi()
+    for _ in range(3):
+        pa, pb = a.iterate(1), b.iterate(1)
+        assert abs(pa - pb) <= 1e-12 * abs(pa)
+    a.close()
+    b.close()
