@@ -173,99 +173,113 @@ struct RowArgs {
 
 // Byte offsets of one row group's current row, stepped with additions.  A
 // unit walks, per item, its k-patches (PK k-rows each, one per row group) and
-// per patch the nj rows.  Offsets split into an item part, a k part (the row
-// group's k-row in the current patch) and a j part.  Before the first row the
-// cursor rests on row 0 of the first patch of the first item; past the last
-// item it keeps the last item, so every offset stays in bounds.
+// per patch the nj rows.  Each stream keeps one combined offset (item, k-row,
+// j-row and the lane's first zone), so a cursor holds few registers.  Before
+// the first row the cursor rests on row 0 of the first patch of the first
+// item; past the last item it keeps the last item, so every offset stays in
+// bounds.  Item offsets are relative to the unit's first item.
 struct RowCursor {
-  int32_t pos;          // rows since the unit's first row (negative: not started)
-  int32_t jj, kp;       // row within the patch (sweep order), k-patch
-  uint32_t j8, jrow8;   // 8*j, 8*j*ni
-  uint32_t zi8, si8;    // 8*item*nz (relative to the unit's first item), 8*g*nz
-  uint32_t fi8[3];      // 8*item*nf per axis (relative)
-  uint32_t kz8;         // 8*(k*nj*ni + lane's first zone)
-  uint32_t kx8, ky8;    // 8*k*nj (i-plane), 8*(k*ni + lane's first zone) (j-plane)
-  int32_t d, g;
-  int32_t bco;          // d*(ni+nj+nk+1): the item's coefficient record in LDS
+  int32_t pos;      // rows since the unit's first row (negative: not started)
+  int32_t jj, kp;   // row within the patch (sweep order), k-patch
+  int32_t inner;    // inner pair index: g when the group is the inner index, else d
+  int32_t bco;      // d*(ni+nj+nk+1): the item's coefficient record in LDS
+  uint32_t zo8;     // rhs/psi: 8*(item*nz + (k*nj + j)*ni + first zone)
+  uint32_t so8;     // sigma_t: 8*(g*nz + (k*nj + j)*ni + first zone)
+  uint32_t vo8;     // zone volume: 8*((k*nj + j)*ni + first zone)
+  uint32_t xo8;     // i-plane face: 8*(item*nf0 + k*nj + j)
+  uint32_t yo8;     // j-plane face: 8*(item*nf1 + k*ni + first zone)
+  uint32_t fo8;     // k-plane face: 8*(item*nf2 + j*ni + first zone)
 };
 
 struct RowConst {
   uint32_t nz8, nf8[3];
   int32_t L;
-  uint32_t j08, jrow08;            // first row in sweep order
   int32_t jstep8, jrowstep8;       // per-row increments
   int32_t NPK;                     // k-patches per item
   int32_t dkz8, dkx8, dky8;        // offset steps from one k-patch to the next
   int32_t rows;                    // R*NPK*nj
 };
 
+// kz8 = 8*(k*nj*ni + first zone), kx8 = 8*k*nj, ky8 = 8*(k*ni + first zone)
+// for the row group's k-row in the first patch; mb8 = 8*first zone.
 __device__ __forceinline__ void cursor_init(RowCursor &c, const RowArgs &P, const RowConst &K, int pos,
-                                            uint32_t kz8, uint32_t kx8, uint32_t ky8) {
+                                            uint32_t kz8, uint32_t kx8, uint32_t ky8, uint32_t mb8) {
   c.pos = pos;
   c.jj = 0;
   c.kp = 0;
-  c.j8 = K.j08;
-  c.jrow8 = K.jrow08;
+  const int j0 = P.jd > 0 ? 0 : P.nj - 1;
+  const uint32_t j08 = 8u * (uint32_t)j0, jrow08 = j08 * (uint32_t)P.ni;
   const int p = P.pg_begin;
+  int d, g;
   if (P.inner_is_g) {
-    c.d = p / P.ng;
-    c.g = p - c.d * P.ng;
+    d = p / P.ng;
+    g = p - d * P.ng;
+    c.inner = g;
   } else {
-    c.g = p / P.nd;
-    c.d = p - c.g * P.nd;
+    g = p / P.nd;
+    d = p - g * P.nd;
+    c.inner = d;
   }
-  // psi/rhs and plane offsets are relative to the unit's first item (the
-  // resources are based there), so they stay below 2 GiB for any chunk size
-  c.zi8 = 0;
-  c.si8 = (uint32_t)c.g * K.nz8;
-#pragma unroll
-  for (int x = 0; x < 3; ++x) c.fi8[x] = 0;
-  c.kz8 = kz8;
-  c.kx8 = kx8;
-  c.ky8 = ky8;
-  c.bco = c.d * K.L;
+  c.bco = d * K.L;
+  c.zo8 = kz8 + jrow08;
+  c.so8 = (uint32_t)g * K.nz8 + kz8 + jrow08;
+  c.vo8 = kz8 + jrow08;
+  c.xo8 = kx8 + j08;
+  c.yo8 = ky8;
+  c.fo8 = mb8 + jrow08;
 }
 
 __device__ __forceinline__ void cursor_step(RowCursor &c, const RowArgs &P, const RowConst &K) {
   if (++c.pos <= 0) return;
   if (++c.jj < P.nj) {
-    c.j8 += K.jstep8;
-    c.jrow8 += K.jrowstep8;
+    c.zo8 += K.jrowstep8;
+    c.so8 += K.jrowstep8;
+    c.vo8 += K.jrowstep8;
+    c.fo8 += K.jrowstep8;
+    c.xo8 += K.jstep8;
     return;
   }
+  // back to the patch's first row
   c.jj = 0;
-  c.j8 = K.j08;
-  c.jrow8 = K.jrow08;
+  const int32_t jback = (P.nj - 1) * K.jrowstep8;
+  c.zo8 -= jback;
+  c.so8 -= jback;
+  c.vo8 -= jback;
+  c.fo8 -= jback;
+  c.xo8 -= (P.nj - 1) * K.jstep8;
   if (c.pos >= K.rows) return;  // past the last item: stay on it
   if (++c.kp < K.NPK) {         // next k-patch of the same item
-    c.kz8 += K.dkz8;
-    c.kx8 += K.dkx8;
-    c.ky8 += K.dky8;
+    c.zo8 += K.dkz8;
+    c.so8 += K.dkz8;
+    c.vo8 += K.dkz8;
+    c.xo8 += K.dkx8;
+    c.yo8 += K.dky8;
     return;
   }
   c.kp = 0;                     // first k-patch of the next item
-  c.kz8 -= (K.NPK - 1) * K.dkz8;
-  c.kx8 -= (K.NPK - 1) * K.dkx8;
-  c.ky8 -= (K.NPK - 1) * K.dky8;
-  c.zi8 += K.nz8;
-#pragma unroll
-  for (int x = 0; x < 3; ++x) c.fi8[x] += K.nf8[x];
+  const int32_t kbz = (K.NPK - 1) * K.dkz8;
+  c.zo8 += K.nz8 - kbz;
+  c.so8 -= kbz;
+  c.vo8 -= kbz;
+  c.xo8 += K.nf8[0] - (K.NPK - 1) * K.dkx8;
+  c.yo8 += K.nf8[1] - (K.NPK - 1) * K.dky8;
+  c.fo8 += K.nf8[2];
   if (P.inner_is_g) {
-    if (++c.g == P.ng) {
-      c.g = 0;
-      ++c.d;
+    if (++c.inner == P.ng) {
+      c.inner = 0;
       c.bco += K.L;
-      c.si8 = 0;
+      c.so8 -= (P.ng - 1) * K.nz8;
     } else {
-      c.si8 += K.nz8;
+      c.so8 += K.nz8;
     }
   } else {
-    if (++c.d == P.nd) {
-      c.d = 0;
-      ++c.g;
-      c.si8 += K.nz8;
+    if (++c.inner == P.nd) {
+      c.inner = 0;
+      c.bco = 0;
+      c.so8 += K.nz8;
+    } else {
+      c.bco += K.L;
     }
-    c.bco = c.d * K.L;
   }
 }
 
@@ -296,9 +310,6 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
 #pragma unroll
   for (int x = 0; x < 3; ++x) K.nf8[x] = 8 * nf[x];
   K.L = ni + nj + nk + 1;  // record: A[ni], B[nj], C[nk], w
-  const int j0 = P.jd > 0 ? 0 : nj - 1;
-  K.j08 = 8 * j0;
-  K.jrow08 = 8 * j0 * ni;
   K.jstep8 = P.jd > 0 ? 8 : -8;
   K.jrowstep8 = K.jstep8 * ni;
   K.NPK = nk / PK;
@@ -360,26 +371,25 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
 
   RowCursor lc, cc;
   cursor_init(lc, P, K, -kk, 8u * (uint32_t)(k * nj * ni + mb), 8u * (uint32_t)(k * nj),
-              8u * (uint32_t)(k * ni + mb));
+              8u * (uint32_t)(k * ni + mb), mb8);
   cc = lc;
 
   auto refill = [&](int r) {
-    const uint32_t zrow = lc.kz8 + lc.jrow8;
-    load_v<V, NT>(rq[r], Rrhs, lc.zi8 + zrow);
-    load_v<V, 0>(rs[r], Rsig, lc.si8 + zrow);
+    load_v<V, NT>(rq[r], Rrhs, lc.zo8);
+    load_v<V, 0>(rs[r], Rsig, lc.so8);
     // faces: the k-face only for row group 0, the j-face only on a patch's first row,
     // the x-face only for lane 0
-    load_v<V, 0>(rz[r], Rown2, (kk == 0 && (lc.kp > 0 || has2)) ? lc.fi8[2] + mb8 + lc.jrow8 : NOWHERE);
-    load_v<V, 0>(ry[r], Rin[1], lc.jj == 0 ? lc.fi8[1] + lc.ky8 : NOWHERE);
+    load_v<V, 0>(rz[r], Rown2, (kk == 0 && (lc.kp > 0 || has2)) ? lc.fo8 : NOWHERE);
+    load_v<V, 0>(ry[r], Rin[1], lc.jj == 0 ? lc.yo8 : NOWHERE);
     {
       double t[1];
-      load_v<1, 0>(t, Rin[0], l == 0 ? lc.fi8[0] + lc.kx8 + lc.j8 : NOWHERE);
+      load_v<1, 0>(t, Rin[0], l == 0 ? lc.xo8 : NOWHERE);
       rx[r] = t[0];
     }
     cursor_step(lc, P, K);
   };
 
-  if (POP) load_v<V, 0>(rv, Rvol, cc.kz8 + cc.jrow8);
+  if (POP) load_v<V, 0>(rv, Rvol, cc.vo8);
 #pragma unroll
   for (int r = 0; r < RING; ++r) refill(r);
 
@@ -395,7 +405,7 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
       const int t = t0 + r;
       const int cur = t & 1, prev = cur ^ 1;
       const bool act = cc.pos >= 0 && cc.pos < K.rows;
-      const double bj = co[cc.bco + ni + (int)(cc.j8 >> 3)];
+      const double bj = co[cc.bco + ni + (P.jd > 0 ? cc.jj : nj - 1 - cc.jj)];
       const int kc = P.kd > 0 ? cc.kp * PK + kk : nk - 1 - cc.kp * PK - kk;  // this row group's k-row
       const double ck = co[cc.bco + ni + nj + kc];
       const double bc = bj + ck;
@@ -476,19 +486,19 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
         for (int e = 0; e < V; ++e) t += (wd * p[e]) * rv[e];
         pacc += st ? t : 0.0;
       }
-      store_v<V, NT>(Rpsi, st ? cc.zi8 + cc.kz8 + cc.jrow8 : NOWHERE, p);
-      store_v<V, 0>(Rout[1], (st && cc.jj == nj - 1) ? cc.fi8[1] + cc.ky8 : NOWHERE, fyo);
+      store_v<V, NT>(Rpsi, st ? cc.zo8 : NOWHERE, p);
+      store_v<V, 0>(Rout[1], (st && cc.jj == nj - 1) ? cc.yo8 : NOWHERE, fyo);
       const bool ktop = st && kk == PK - 1, klast = cc.kp == K.NPK - 1;
-      store_v<V, 0>(Rout[2], (ktop && klast) ? cc.fi8[2] + mb8 + cc.jrow8 : NOWHERE, fzo);
-      store_v<V, 0>(Rown2, (ktop && !klast) ? cc.fi8[2] + mb8 + cc.jrow8 : NOWHERE, fzo);
+      store_v<V, 0>(Rout[2], (ktop && klast) ? cc.fo8 : NOWHERE, fzo);
+      store_v<V, 0>(Rown2, (ktop && !klast) ? cc.fo8 : NOWHERE, fzo);
       {
         const double t[1] = {f};
-        store_v<1, 0>(Rout[0], (act && l == nl - 1) ? cc.fi8[0] + cc.kx8 + cc.j8 : NOWHERE, t);
+        store_v<1, 0>(Rout[0], (act && l == nl - 1) ? cc.xo8 : NOWHERE, t);
       }
 #pragma unroll
       for (int m = 0; m < V; ++m) fzb[(cur * nt + tid) * V + m] = fzo[memo<SX, V>(m)];
       cursor_step(cc, P, K);
-      if (POP) load_v<V, 0>(rv, Rvol, cc.kz8 + cc.jrow8);
+      if (POP) load_v<V, 0>(rv, Rvol, cc.vo8);
       refill(r);
       __syncthreads();
     }
