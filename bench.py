@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch leg")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-launch leg and print its record (for profiling that leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -269,6 +271,10 @@ def main():
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs as a single process on one GPU", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch_only:
+        print(json.dumps({"batched": run_batched()}))
+        return
 
     elapsed, setup_s, timers, nb = run_gpu(args, args.layout)
     nz = 64 ** 3
