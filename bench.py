@@ -118,12 +118,17 @@ def run_adjoint(layout):
     for b, v in zip(bs, sigt):
         chunk_set(ctx, b.ptr.value, 0.5 * v)
     sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
-    # detector box: 4^3 zones at a seeded corner; target phi = 0.9 x the phi of the unperturbed solve
+    # detector box: 4^3 zones at a seeded corner; target phi = the phi of a solve with sigma_t perturbed by a
+    # seeded +-20 % per zone and group
     nx = ADJ_WORKLOAD["zones"][0]
     c0 = rng.integers(0, nx - 4, size=3)
+    bp = [DeviceBuffer(ctx, n) for n in sv.nparam]
+    for b, v in zip(bp, sigt):
+        chunk_set(ctx, b.ptr.value, v * (1.0 + 0.2 * (2.0 * rng.random(v.size) - 1.0)))
+    sv.set_params([b.ptr.value for b in bp], [b.ptr.value for b in bs])
     sv.forward()
-    phi0 = [chunk_get(ctx, p, n) for p, n in zip(sv.phi_ptrs(), sv.nphi)]
-    target = [0.9 * v for v in phi0]
+    target = [chunk_get(ctx, p, n) for p, n in zip(sv.phi_ptrs(), sv.nphi)]
+    sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
     M, G = 9, ADJ_WORKLOAD["groups"]
     mask = np.zeros((nx, nx, nx), dtype=bool)
     mask[c0[0]:c0[0] + 4, c0[1]:c0[1] + 4, c0[2]:c0[2] + 4] = True
@@ -149,16 +154,99 @@ def run_adjoint(layout):
         ctx.synchronize()
         times["backward"].append(time.perf_counter() - t0)
     g = chunk_get(ctx, sv.param_ptrs("grad_sigt")[0], sv.nparam[0])
-    for b in bt + bs + wb:
+    for b in bt + bs + wb + bp:
         b.free()
     sv.close()
     ctx.close()
     tf, tb = min(times["forward"]), min(times["backward"])
     return {"workload": "32^3 zones, 32 groups, 96 directions GL 8x12, P2 (9 moments), %s, 10 iterations from "
                         "psi=0, phi checkpoints every 2; parameters: sigma_t and sigma_s per zone and group; loss on "
-                        "a seeded 4^3 detector box" % layout,
+                        "a seeded 4^3 detector box against phi of a seeded +-20%% sigma_t perturbation" % layout,
             "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "adjoint_forward_cost": tb / tf,
             "loss": loss, "grad_sigt_norm": float(np.linalg.norm(g))}
+
+
+ADJB_WORKLOAD = dict(zones=(48, 48, 48), groups=64, quad=(8, 16), legendre=3, dset=8, gset=1, zset=(1, 1, 1),
+                     geometry=1, seed=20261203)
+ADJB_FIELDS, ADJB_ITERATIONS, ADJB_CHECKPOINT = 8, 20, 4
+
+
+def run_adjoint_batched(layout):
+    """Gradient throughput over candidate cross-section fields: 8 fields of
+    48^3 zones x 64 groups x 128 directions (GL 8x16), P3, 20 source
+    iterations from psi = 0 with phi checkpoints every 4.  Field f has
+    sigma_t(g,z) x (1 + 0.2 u_f) (u_f seeded uniform in [-1, 1]) and
+    sigma_s = 0.5 sigma_t; the loss is the squared mismatch of the angular flux
+    leaving the domain through all 6 faces against the exiting flux of a
+    seeded 'truth' field.  The fields run one after another through one
+    differentiable solver (all 8 at once would hold about 300 GB)."""
+    import numpy as np
+    import kripke_amd as ka
+    from kripke_amd.adjoint import DeviceBuffer, DifferentiableSolver, chunk_get, chunk_set
+    ctx = ka.Context(0)
+    sv = DifferentiableSolver(ctx, ka.input_vars(layout=layout, **ADJB_WORKLOAD), ADJB_ITERATIONS, ADJB_CHECKPOINT)
+    sv.kf.autotune()
+    sv.ka.autotune()
+    rng = np.random.default_rng(20261203)
+    sigt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("sigt"), sv.nparam)]
+    bt = [DeviceBuffer(ctx, n) for n in sv.nparam]
+    bs = [DeviceBuffer(ctx, n) for n in sv.nparam]
+    exits = sv.exit_faces()
+
+    def set_field(scale):
+        for b, c, v in zip(bt, bs, sigt):
+            pert = v * scale[:v.size] if scale is not None else v
+            chunk_set(ctx, b.ptr.value, pert)
+            chunk_set(ctx, c.ptr.value, 0.5 * pert)
+        sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+
+    nparam = max(sv.nparam)
+    set_field(1.0 + 0.2 * (2.0 * rng.random(nparam) - 1.0))  # the truth field
+    sv.forward()
+    target = {(sd, nm): chunk_get(ctx, p, n) for sd, nm, p, n in exits}
+    fb = {(sd, nm): DeviceBuffer(ctx, n) for sd, nm, _, n in exits}
+    scales = [1.0 + 0.2 * (2.0 * rng.random(nparam) - 1.0) for _ in range(ADJB_FIELDS)]
+    tf = tb = 0.0
+    losses = []
+    t_all = time.perf_counter()
+    for f in range(ADJB_FIELDS + 1):  # field 0 once more first, untimed (warm-up)
+        set_field(scales[max(f - 1, 0)])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sv.forward()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        loss = 0.0
+        for sd, nm, p, n in exits:
+            d = chunk_get(ctx, p, n) - target[(sd, nm)]
+            loss += float(np.dot(d, d))
+            chunk_set(ctx, fb[(sd, nm)].ptr.value, 2.0 * d)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        sv.backward(None, {key: b.ptr.value for key, b in fb.items()})
+        ctx.synchronize()
+        t3 = time.perf_counter()
+        if f == 0:
+            t_all = time.perf_counter()
+            continue
+        tf += t1 - t0
+        tb += t3 - t2
+        losses.append(loss)
+    wall = time.perf_counter() - t_all
+    g = chunk_get(ctx, sv.param_ptrs("grad_sigt")[0], sv.nparam[0])
+    for b in bt + bs + list(fb.values()):
+        b.free()
+    sv.close()
+    ctx.close()
+    return {"workload": "%d fields x (48^3 zones, 64 groups, 128 directions GL 8x16, P3 (16 moments), %s), %d "
+                        "iterations from psi=0, phi checkpoints every %d; parameters sigma_t and sigma_s per zone "
+                        "and group; loss on the exiting angular flux of all 6 boundary faces"
+                        % (ADJB_FIELDS, layout, ADJB_ITERATIONS, ADJB_CHECKPOINT),
+            "fields_per_s": ADJB_FIELDS / (tf + tb), "fields_per_s_wall": ADJB_FIELDS / wall,
+            "forward_ms_per_field": tf / ADJB_FIELDS * 1e3, "backward_ms_per_field": tb / ADJB_FIELDS * 1e3,
+            "adjoint_forward_cost": tb / tf, "losses": losses, "grad_sigt_norm_last": float(np.linalg.norm(g)),
+            "note": "fields_per_s counts the forward and backward passes; fields_per_s_wall adds the host-side "
+                    "loss and cotangent (exiting faces copied to the host)"}
 
 
 BATCH_PROBLEMS, BATCH_ITERATIONS = 48, 15
@@ -323,6 +411,7 @@ def main():
     }
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint(args.layout)
+        out["adjoint_batched"] = run_adjoint_batched(args.layout)
     if not args.no_batch:
         out["batched"] = run_batched()
     if not args.no_cpu:

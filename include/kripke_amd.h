@@ -195,6 +195,20 @@ int kr_scattering_adjoint(kr_problem *prob, int32_t accumulate);
 int kr_scattering_zonal(kr_problem *prob, const double *const *sigs_zonal, int32_t accumulate);
 int kr_scattering_zonal_adjoint(kr_problem *prob, const double *const *sigs_zonal, int32_t accumulate);
 int kr_sweep_solver_adjoint(kr_problem *prob);
+/* No counterpart in the reference.  kr_sweep_solver_adjoint with adjoint
+ * inflow on the domain boundary: where the forward flow leaves the domain
+ * (kr_sdom_exits), the incoming adjoint faces are read from the subdomain's
+ * own plane chunk ("i_plane"/"j_plane"/"k_plane"), which the caller fills
+ * with the cotangent of the exiting faces.  After a forward
+ * kr_sweep_solver, those chunks of the forward problem hold the exiting
+ * faces themselves (SweepSolver.cpp:71-75: sweepSubdomain writes the
+ * outgoing faces into the subdomain's planes and no downwind subdomain
+ * takes them), so a loss on the exiting angular flux differentiates
+ * through this call. */
+int kr_sweep_solver_adjoint_seeded(kr_problem *prob);
+/* exits[a] = 1 when subdomain sdom's downwind face along axis a (0 = i,
+ * 1 = j, 2 = k) is the domain boundary (no downwind subdomain). */
+int kr_sdom_exits(kr_problem *prob, int32_t sdom, int32_t *exits);
 int kr_grad_sigt(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
 int kr_grad_sigs_zonal(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
 

@@ -18,8 +18,12 @@ the (small) phi_n.
 Backward, for a cotangent phibar_K of phi_K:
 
     psibar_K = L^T phibar_K
-    for n = K-1 .. 0:   (phi_n and psi_{n+1} recomputed from the nearest checkpoint)
+    for n = K-1 .. 0:   (phi_n and psi_{n+1} recomputed: the phi of a whole
+                         checkpoint interval once from its checkpoint, then
+                         psi_{n+1} from phi_n)
         rhsbar_n     = H^-T psibar_{n+1}                 adjoint sweep
+                       (n = K-1: with the cotangent of the last sweep's
+                       exiting boundary faces as adjoint inflow)
         d sigt      += -sum_{d} rhsbar_n * psi_{n+1}
         phi_outbar_n = (L+)^T rhsbar_n
         d s         += phi_outbar_n(0) * phi_n(0)
@@ -60,6 +64,9 @@ def _fill(ctx, dst, n, v):
     check(lib().kr_kconst(ctx.h, ctypes.c_void_p(dst), int(n), float(v)), "kr_kconst")
 
 
+PLANES = ("i_plane", "j_plane", "k_plane")
+
+
 class DifferentiableSolver:
     """K source iterations as a function of (sigt, s) with phi checkpoints
     every `checkpoint_every` iterations."""
@@ -81,6 +88,8 @@ class DifferentiableSolver:
         self._gs = [DeviceBuffer(ctx, n) for n in self.nparam]       # gradient w.r.t. s
         nck = (self.K + self.ck - 1) // self.ck
         self._ckpt = [[DeviceBuffer(ctx, n) for n in self.nphi] for _ in range(nck)]
+        # phi of the other iterations of one checkpoint interval, recomputed once per interval in backward()
+        self._blk = [[DeviceBuffer(ctx, n) for n in self.nphi] for _ in range(self.ck - 1)]
         self.forward_done = False
 
     # ---- parameters ------------------------------------------------------
@@ -135,25 +144,62 @@ class DifferentiableSolver:
         hp = self.fwd if problem == "fwd" else self.adj
         return [hp.device_ptr("phi", c)[0] for c in range(self.npr)]
 
+    def exit_faces(self):
+        """(sdom, plane name, device address, count) of every plane chunk of
+        the forward problem that holds faces leaving the domain; after
+        forward() they hold the last sweep's exiting angular flux."""
+        out = []
+        for s in self.sdoms:
+            for a, leaves in enumerate(self.kf.sdom_exits(s)):
+                if leaves:
+                    ptr, n = self.fwd.device_ptr(PLANES[a], s)
+                    out.append((s, PLANES[a], ptr, n))
+        return out
+
     # ---- backward --------------------------------------------------------
-    def backward(self, phibar_ptrs):
-        """Gradient of <phibar, phi_K> w.r.t. sigt and s, left in
-        param_ptrs('grad_sigt') and param_ptrs('grad_s')."""
+    def backward(self, phibar_ptrs=None, facebar=None):
+        """Gradient of <phibar, phi_K> + <facebar, exiting faces of the last
+        sweep> w.r.t. sigt and s, left in param_ptrs('grad_sigt') and
+        param_ptrs('grad_s').  phibar_ptrs: one device array per PR chunk (None:
+        zero); facebar: {(sdom, plane name): device array} over exit_faces()."""
         if not self.forward_done:
             raise RuntimeError("backward needs the forward pass of the same parameters")
+        exits = {(s, name): n for s, name, _, n in self.exit_faces()}
+        for key in (facebar or {}):
+            if key not in exits:
+                raise ValueError("facebar names a plane chunk that does not leave the domain: %r" % (key,))
         for c in range(self.npr):
-            _copy(self.ctx, self.adj.device_ptr("phi", c)[0], phibar_ptrs[c], self.nphi[c])
+            if phibar_ptrs is None:
+                _fill(self.ctx, self.adj.device_ptr("phi", c)[0], self.nphi[c], 0.0)
+            else:
+                _copy(self.ctx, self.adj.device_ptr("phi", c)[0], phibar_ptrs[c], self.nphi[c])
             _fill(self.ctx, self._gt[c].ptr.value, self.nparam[c], 0.0)
             _fill(self.ctx, self._gs[c].ptr.value, self.nparam[c], 0.0)
         self.ka.ltimes_adjoint()                                  # psibar_K
         gt, gs = self.param_ptrs("grad_sigt"), self.param_ptrs("grad_s")
         for n in range(self.K - 1, -1, -1):
-            c0 = (n // self.ck) * self.ck
-            self._set_phi(self._ckpt[n // self.ck])               # phi_c0
-            for _ in range(c0, n):
-                self._step()                                      # phi_n
+            b, c0 = n // self.ck, (n // self.ck) * self.ck
+            if n == min(self.K, c0 + self.ck) - 1:
+                # last iteration of an interval: recompute phi_{c0+1} .. phi_n once from the checkpoint
+                self._set_phi(self._ckpt[b])
+                for j in range(c0, n):
+                    self._step()
+                    for c in range(self.npr):
+                        _copy(self.ctx, self._blk[j - c0][c].ptr.value, self.fwd.device_ptr("phi", c)[0],
+                              self.nphi[c])
+            else:
+                self._set_phi(self._ckpt[b] if n == c0 else self._blk[n - c0 - 1])  # phi_n
             self._tail()                                          # psi_{n+1}; phi still phi_n
-            self.ka.sweep_solver_adjoint()                        # rhsbar_n
+            if n == self.K - 1 and facebar:
+                for (s, name), n_el in exits.items():             # adjoint inflow: the faces' cotangent
+                    dst = self.adj.device_ptr(name, s)[0]
+                    if (s, name) in facebar:
+                        _copy(self.ctx, dst, facebar[(s, name)], n_el)
+                    else:
+                        _fill(self.ctx, dst, n_el, 0.0)
+                self.ka.sweep_solver_adjoint(seeded=True)         # rhsbar_{K-1}
+            else:
+                self.ka.sweep_solver_adjoint()                    # rhsbar_n
             self.kf.grad_sigt(self.ka, gt, accumulate=True)
             self.ka.lplustimes_adjoint()                          # phi_outbar_n
             self.kf.grad_sigs_zonal(self.ka, gs, accumulate=True)
@@ -166,7 +212,7 @@ class DifferentiableSolver:
     def close(self):
         for b in self._s + self._gt + self._gs:
             b.free()
-        for ck in self._ckpt:
+        for ck in self._ckpt + self._blk:
             for b in ck:
                 b.free()
         self.fwd.close()

@@ -112,6 +112,8 @@ SIGNATURES = [
     ("kr_scattering_zonal", ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_scattering_zonal_adjoint", ctypes.c_int, [P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_sweep_solver_adjoint", ctypes.c_int, [P]),
+    ("kr_sweep_solver_adjoint_seeded", ctypes.c_int, [P]),
+    ("kr_sdom_exits", ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     ("kr_grad_sigt", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_grad_sigs_zonal", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_problem_autotune", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32)]),
@@ -264,8 +266,20 @@ class KernelProblem:
         fn = lib().kr_scattering_zonal_adjoint if adjoint else lib().kr_scattering_zonal
         check(fn(self.h, arr, 1 if accumulate else 0), "kr_scattering_zonal")
 
-    def sweep_solver_adjoint(self):
-        check(lib().kr_sweep_solver_adjoint(self.h), "kr_sweep_solver_adjoint")
+    def sweep_solver_adjoint(self, seeded=False):
+        """rhs = H^-T psi; seeded: with adjoint inflow on the domain boundary
+        read from the own plane chunks (kr_sweep_solver_adjoint_seeded)."""
+        if seeded:
+            check(lib().kr_sweep_solver_adjoint_seeded(self.h), "kr_sweep_solver_adjoint_seeded")
+        else:
+            check(lib().kr_sweep_solver_adjoint(self.h), "kr_sweep_solver_adjoint")
+
+    def sdom_exits(self, sdom):
+        """(i, j, k) flags: the subdomain's downwind face on that axis is the
+        domain boundary."""
+        e = (ctypes.c_int32 * 3)()
+        check(lib().kr_sdom_exits(self.h, int(sdom), e), "kr_sdom_exits")
+        return tuple(bool(v) for v in e)
 
     def grad_sigt(self, adj, ptrs, accumulate=False):
         arr = (P * len(ptrs))(*ptrs)

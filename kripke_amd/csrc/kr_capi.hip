@@ -65,6 +65,7 @@ struct kr_problem {
   double *d_pop_units = nullptr;               // fused population: one partial per sweep unit
   int64_t pop_units_cap = 0;
   SweepPlanes *d_planes_adj = nullptr;         // adjoint sweep: faces from the forward-downwind side
+  SweepPlanes *d_planes_adj_seed = nullptr;    // the same, with inflow on the domain boundary from the own planes
   SweepPlanes *d_planes_bj = nullptr;          // Block Jacobi: no incoming faces, outgoing faces kept in place
   static constexpr int PTR_SLOTS = 4;          // zonal sigma_s, sigma_t gradient, sigma_s gradient, spare
   const double **d_ptrs = nullptr;             // PTR_SLOTS per-PR-chunk pointer arrays passed to kernels
@@ -570,7 +571,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   }
 
   // --- sweep planes tables --------------------------------------------------
-  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom), padj(p->nsdom), pbj(p->nsdom);
+  std::vector<SweepPlanes> pref(p->nsdom), ppipe(p->nsdom), padj(p->nsdom), pseed(p->nsdom), pbj(p->nsdom);
   for (int s = 0; s < p->nsdom; ++s) {
     const DevSdom &H = p->h_sd[s];
     for (int dim = 0; dim < 3; ++dim) {
@@ -582,6 +583,10 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
       // forward-downwind neighbour and leave toward the forward-upwind one
       padj[s].in[dim] = H.down[dim] >= 0 ? H.plane[dim] : nullptr;
       padj[s].out[dim] = H.up[dim] >= 0 ? p->h_sd[H.up[dim]].plane[dim] : H.plane[dim];
+      // seeded: where the forward flow leaves the domain, the adjoint inflow
+      // is read from the own plane chunk (the caller's exiting-face cotangent)
+      pseed[s].in[dim] = H.plane[dim];
+      pseed[s].out[dim] = padj[s].out[dim];
       pbj[s].in[dim] = nullptr;
       pbj[s].out[dim] = H.plane[dim];
     }
@@ -635,6 +640,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   if ((rc = upload(ctx, &p->d_sd, hsd)) || (rc = upload(ctx, &p->d_pr, p->h_pr)) || (rc = upload(ctx, &p->d_r, hr)) ||
       (rc = upload(ctx, &p->d_pr_sdoms, pr_sdoms)) || (rc = upload(ctx, &p->d_planes_ref, pref)) ||
       (rc = upload(ctx, &p->d_planes_pipe, ppipe)) || (rc = upload(ctx, &p->d_planes_adj, padj)) ||
+      (rc = upload(ctx, &p->d_planes_adj_seed, pseed)) ||
       (rc = upload(ctx, &p->d_planes_bj, pbj)) ||
       (rc = upload(ctx, &p->d_units_ref, uref)) ||
       (rc = upload(ctx, &p->d_units_pipe, upipe))) {
@@ -665,7 +671,7 @@ int kr_problem_destroy(kr_problem *p) {
   void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
                   p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
                   p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units,
-                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t, p->d_planes_bj};
+                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t, p->d_planes_bj, p->d_planes_adj_seed};
   for (void *b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -928,17 +934,32 @@ int kr_scattering_zonal_adjoint(kr_problem *p, const double *const *sigs_zonal, 
   return launch_scattering_zonal(launch_ctx(p), dev, sg, sz, accumulate ? 1 : 0, true);
 }
 
-int kr_sweep_solver_adjoint(kr_problem *p) {
-  if (!p) return fail(KR_ERR_INVALID, "kr_sweep_solver_adjoint: null problem");
+static int sweep_solver_adjoint_on(kr_problem *p, const SweepPlanes *planes) {
   CallTimer t(p, "SweepSolverAdjoint");
   int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, 0, p->nsdom);
   if (rc) return rc;
   for (size_t n = p->pipe_level_begin.size(); n-- > 0;) {  // the forward stages in reverse order
     CallTimer tk(p, "k_sweep_adjoint");
-    rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_adj, p->d_units_pipe + p->pipe_level_begin[n],
+    rc = launch_sweep(p->ctx->stream, p->d_sd, planes, p->d_units_pipe + p->pipe_level_begin[n],
                       p->pipe_level_count[n], p->shape_pipe, nullptr, true);
     if (rc) return rc;
   }
+  return KR_OK;
+}
+
+int kr_sweep_solver_adjoint(kr_problem *p) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_sweep_solver_adjoint: null problem");
+  return sweep_solver_adjoint_on(p, p->d_planes_adj);
+}
+
+int kr_sweep_solver_adjoint_seeded(kr_problem *p) {
+  if (!p) return fail(KR_ERR_INVALID, "kr_sweep_solver_adjoint_seeded: null problem");
+  return sweep_solver_adjoint_on(p, p->d_planes_adj_seed);
+}
+
+int kr_sdom_exits(kr_problem *p, int32_t sdom, int32_t *exits) {
+  if (!p || !exits || sdom < 0 || sdom >= p->nsdom) return fail(KR_ERR_INVALID, "kr_sdom_exits: bad argument");
+  for (int dim = 0; dim < 3; ++dim) exits[dim] = p->h_sd[sdom].down[dim] < 0 ? 1 : 0;
   return KR_OK;
 }
 

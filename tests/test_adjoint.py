@@ -145,6 +145,38 @@ def test_sweep_transpose(ctx, layout, zones):
     _check_dot((y, Hx), (Hty, x))
 
 
+PLANES = ("i_plane", "j_plane", "k_plane")
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("zones,zset", [((12, 6, 10), (2, 2, 2)), ((8, 8, 8), (1, 1, 1)), ((128, 80, 128), (2, 2, 2))])
+def test_sweep_transpose_seeded(ctx, layout, zones, zset):
+    """The sweep as a map rhs -> (psi, faces leaving the domain) and its
+    transpose, the seeded adjoint sweep (the faces' cotangent enters as
+    adjoint inflow on the domain boundary):
+    <y, H x> + <fb, B x> = <H^T y + B^T fb, x>."""
+    big = zones[0] > 64
+    hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=2 if big else 4, zones=zones,
+                                       quad=(4, 2) if big else (4, 4), legendre=1))
+    k = hp.kernels()
+    rng = np.random.default_rng(47)
+    exits = [(s, PLANES[a]) for s in range(hp.num_sdoms) for a, e in enumerate(k.sdom_exits(s)) if e]
+    assert exits
+    x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.sweep_solver(list(range(hp.num_sdoms)))
+    Hx = _get(hp, "psi", hp.num_sdoms)
+    Bx = [hp.get(name, s) for s, name in exits]
+    y = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    fb = []
+    for s, name in exits:
+        v = rng.standard_normal(hp.get(name, s).size)
+        hp.set(name, s, v)
+        fb.append(v)
+    k.sweep_solver_adjoint(seeded=True)
+    Aty = _get(hp, "rhs", hp.num_sdoms)
+    _check_dot((y + fb, Hx + Bx), (Aty, x))
+
+
 @pytest.mark.parametrize("layout", ["DGZ", "GDZ"])
 @pytest.mark.parametrize("zones,zset,quad", [((32, 24, 32), (1, 1, 2), (4, 8)), ((32, 12, 32), (1, 2, 1), (4, 8)),
                                              ((16, 16, 16), (2, 1, 1), (8, 8)), ((128, 6, 8), (1, 1, 2), (4, 4))])
@@ -233,6 +265,57 @@ def test_gradient_vs_finite_differences(ctx, layout):
             assert abs(fd - grad[c][e]) <= tol, \
                 "%s chunk %d entry %d: fd %.12e adjoint %.12e" % (which, c, e, fd, grad[c][e])
     for b in bt + bs + wbufs:
+        b.free()
+    sv.close()
+
+
+@pytest.mark.parametrize("layout", ["DGZ", "DZG"])
+def test_exiting_flux_gradient_vs_finite_differences(ctx, layout):
+    """Loss on the angular flux leaving the domain after K iterations,
+    J = sum over the exiting faces of (psi_exit - target)^2: the adjoint
+    gradient (exiting-face cotangent as adjoint inflow of the last sweep)
+    against central differences of the forward solve."""
+    from kripke_amd.adjoint import DeviceBuffer, chunk_get, chunk_set
+    sv = _solver(ctx, layout, iterations=3, ck=2)
+    sigt, s, bt, bs = _params(sv, ctx)
+    rng = np.random.default_rng(49)
+    exits = sv.exit_faces()
+    target = {(sd, nm): rng.uniform(0.0, 0.1, n) for sd, nm, _, n in exits}
+
+    def loss():
+        return sum(float(np.sum((chunk_get(ctx, p, n) - target[(sd, nm)]) ** 2)) for sd, nm, p, n in exits)
+
+    sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+    sv.forward()
+    J0 = loss()
+    fbufs = {}
+    for sd, nm, p, n in exits:
+        b = DeviceBuffer(ctx, n)
+        chunk_set(ctx, b.ptr.value, 2.0 * (chunk_get(ctx, p, n) - target[(sd, nm)]))
+        fbufs[(sd, nm)] = b
+    sv.backward(None, {key: b.ptr.value for key, b in fbufs.items()})
+    gt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("grad_sigt"), sv.nparam)]
+    gs = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("grad_s"), sv.nparam)]
+    assert any(np.any(g != 0.0) for g in gt)
+    for which, base, grad, bufs in (("sigt", sigt, gt, bt), ("s", s, gs, bs)):
+        for _ in range(6):
+            c = int(rng.integers(len(base)))
+            e = int(rng.integers(base[c].size))
+            h = 1e-6 * abs(base[c][e])
+            vals = []
+            for sgn in (1.0, -1.0):
+                v = base[c].copy()
+                v[e] += sgn * h
+                chunk_set(ctx, bufs[c].ptr.value, v)
+                sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+                sv.forward()
+                vals.append(loss())
+            chunk_set(ctx, bufs[c].ptr.value, base[c])
+            fd = (vals[0] - vals[1]) / (2 * h)
+            tol = 1e-6 * abs(fd) + 1e-13 * J0 / h
+            assert abs(fd - grad[c][e]) <= tol, \
+                "%s chunk %d entry %d: fd %.12e adjoint %.12e" % (which, c, e, fd, grad[c][e])
+    for b in bt + bs + list(fbufs.values()):
         b.free()
     sv.close()
 
