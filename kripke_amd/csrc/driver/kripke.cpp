@@ -29,6 +29,7 @@ struct DriverVars {
   int npx = 1, npy = 1, npz = 1;
   int mode = 1;  // 0: the reference call sequence, 1: on-device pipeline
   int device = 0;
+  bool layout_auto = false;  // --layout auto: kr_host_autotune_nesting picks it
   std::string run_name = "kripke";
   DriverVars() {
     in.nx = in.ny = in.nz = 16;
@@ -85,7 +86,8 @@ struct DriverVars {
   printf("                         Available: HIP\n");
   printf("                         Default:   --arch HIP\n\n");
   printf("  --layout <LAYOUT>      Data layout and loop nesting order\n");
-  printf("                         Available: DGZ,DZG,GDZ,GZD,ZDG,ZGD\n");
+  printf("                         Available: DGZ,DZG,GDZ,GZD,ZDG,ZGD,auto\n");
+  printf("                         (auto: the fastest, timed on the device)\n");
   printf("                         Default:   --layout DGZ\n\n");
   printf("  --device <n>           HIP device\n");
   printf("                         Default:   --device 0\n\n");
@@ -237,6 +239,8 @@ void parse(DriverVars &v, int argc, char **argv) {
       int found = -1;
       for (int i = 0; i < 6; ++i)
         if (!strcasecmp(l.c_str(), LAYOUT_NAMES[i])) found = i;
+      v.layout_auto = !strcasecmp(l.c_str(), "auto");
+      if (v.layout_auto) found = KR_DGZ;
       if (found < 0) {
         printf("Unknown layout %s\n", l.c_str());
         exit(1);
@@ -291,7 +295,7 @@ void print_inputs(const DriverVars &v) {
   printf("    GroupSet/Groups:       %d sets, %d groups/set\n", in.num_groupsets, in.num_groups / in.num_groupsets);
   printf("    Zone Sets:             %d x %d x %d\n", in.num_zonesets[0], in.num_zonesets[1], in.num_zonesets[2]);
   printf("    Architecture:          HIP (device %d)\n", v.device);
-  printf("    Data Layout:           %s\n", LAYOUT_NAMES[in.layout]);
+  printf("    Data Layout:           %s\n", v.layout_auto ? "auto (chosen by the autotuner)" : LAYOUT_NAMES[in.layout]);
 }
 
 // Memory breakdown of the double fields (src/Kripke/Generate.cpp:56-94)
@@ -337,6 +341,17 @@ int main(int argc, char **argv) {
   if (kr_context_create(v.device, &ctx) != KR_OK) {
     printf("ERROR: %s\n", kr_last_error());
     return 1;
+  }
+  if (v.layout_auto) {
+    int32_t nest = KR_DGZ;
+    double ms = 0.0;
+    if (kr_host_autotune_nesting(ctx, &v.in, &nest, &ms) != KR_OK) {
+      printf("ERROR: %s\n", kr_last_error());
+      kr_context_destroy(ctx);
+      return 1;
+    }
+    v.in.layout = nest;
+    printf("\nLayout chosen by the autotuner: %s (%.3f ms per source iteration)\n", LAYOUT_NAMES[nest], ms);
   }
   int rc = 0;
   try {

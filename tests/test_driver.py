@@ -102,3 +102,18 @@ def test_reference_default_problem():
     for g, w in zip(got, want):
         assert abs(g - w) <= 1e-6 * abs(w), (g, w)
     assert "Dummy S2 with 96 points" in out
+
+
+@pytest.mark.gpu
+def test_layout_auto():
+    """--layout auto: the autotuner's nesting is reported and the counts still
+    match the oracle (the nesting changes only the memory order)."""
+    rc, out = run(*(ARGS + ["--layout", "auto"]))
+    assert rc == 0, out[-2000:]
+    m = re.search(r"Layout chosen by the autotuner: (\w+) \((\S+) ms per source iteration\)", out)
+    assert m and m.group(1) in ("DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD") and float(m.group(2)) > 0
+    assert "Data Layout:           auto" in out
+    got = _counts(out)
+    want = _oracle_counts(3, layout=m.group(1), pmethod="sweep", **CFG)
+    for g, w in zip(got, want):
+        assert abs(g - w) <= 1e-6 * abs(w), (g, w)
