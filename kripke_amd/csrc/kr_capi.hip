@@ -1140,6 +1140,7 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
 // value equals the problem's own launches bit for bit; the population of a
 // problem adds its units' partials in its own unit order.
 constexpr int BATCH_BRANCHES = 8;
+constexpr int BATCH_PARTS = 4;  // problem groups advanced on their own graph branches
 
 struct BatchSweep {
   SweepShape shape;     // the members' shape (row-scan kernel: their common RG, V, PK)
@@ -1153,6 +1154,27 @@ struct BatchLt {
   int64_t max_gz = 0;
 };
 
+// The problems of one part: combined tables and launch lists.
+struct BatchPart {
+  int32_t p0 = 0, np = 0;            // problems [p0, p0 + np) of the batch
+  char *d_tab = nullptr;             // the tables, one device block
+  DevSdom *d_sd = nullptr;
+  DevPR *d_pr = nullptr, *d_pr_lt = nullptr, *d_pr_lp = nullptr;
+  DevR *d_r = nullptr;
+  int32_t *d_pr_sdoms = nullptr;
+  SweepPlanes *d_planes = nullptr;
+  SweepUnit *d_units = nullptr;
+  DevScat *d_scat_chunk = nullptr;
+  int64_t *d_pop_list = nullptr, *d_pop_begin = nullptr;
+  double *d_pop_units = nullptr;     // one population partial per combined unit
+  int32_t nsd = 0, npr = 0;
+  int64_t max_gz = 0, max_gmix = 0;
+  std::vector<BatchLt> lt, lp;
+  std::vector<std::vector<BatchSweep>> levels;
+  hipStream_t stream = nullptr;      // the part's graph branch while capturing
+  hipEvent_t done = nullptr;
+};
+
 struct kr_batch {
   kr_context *ctx = nullptr;
   std::vector<kr_problem *> probs;
@@ -1164,22 +1186,9 @@ struct kr_batch {
   std::vector<int32_t> coupled;      // per problem: its sigs couples groups or moments (at capture)
   double *d_pop = nullptr;           // [problem][iteration]
   int64_t pop_cap = 0;
-  // combined tables (one device block)
-  char *d_tab = nullptr;
-  DevSdom *d_sd = nullptr;
-  DevPR *d_pr = nullptr, *d_pr_lt = nullptr, *d_pr_lp = nullptr;
-  DevR *d_r = nullptr;
-  int32_t *d_pr_sdoms = nullptr;
-  SweepPlanes *d_planes = nullptr;
-  SweepUnit *d_units = nullptr;
-  DevScat *d_scat_prob = nullptr, *d_scat_chunk = nullptr;
   int32_t *d_flags = nullptr;        // SIGS_FLAG_BLOCKS per problem
-  int64_t *d_pop_list = nullptr, *d_pop_begin = nullptr;
-  double *d_pop_units = nullptr;     // one population partial per combined unit
-  int32_t nsd = 0, npr = 0;
-  int64_t max_gz = 0, max_gmix = 0;
-  std::vector<BatchLt> lt, lp;
-  std::vector<std::vector<BatchSweep>> levels;
+  DevScat *d_scat_all = nullptr;     // one entry per problem (structure check)
+  std::vector<BatchPart> parts;
   hipStream_t branch[BATCH_BRANCHES] = {};
   hipEvent_t join[BATCH_BRANCHES] = {};
   hipEvent_t fork = nullptr;
@@ -1191,8 +1200,14 @@ struct kr_batch {
 // s0 and joined back (graph branches when captured): independent launches of
 // one step may overlap, so the tail of one fills with the start of another.
 template <class F>
-static int batch_parallel(kr_batch *b, hipStream_t s0, int n, F &&launch) {
-  if (n <= 1) return n == 1 ? launch(s0, 0) : KR_OK;
+static int batch_parallel(kr_batch *b, hipStream_t s0, int n, bool branches, F &&launch) {
+  if (n <= 1 || !branches) {
+    for (int i = 0; i < n; ++i) {
+      int rc = launch(s0, i);
+      if (rc) return rc;
+    }
+    return KR_OK;
+  }
   KR_HIP_CHECK(hipEventRecord(b->fork, s0));
   const int used = std::min(n, BATCH_BRANCHES);
   for (int k = 0; k < used; ++k) KR_HIP_CHECK(hipStreamWaitEvent(b->branch[k], b->fork, 0));
@@ -1210,13 +1225,15 @@ static int batch_parallel(kr_batch *b, hipStream_t s0, int n, F &&launch) {
 extern "C" {
 
 static void batch_free_tables(kr_batch *b) {
-  if (b->d_tab) hipFree(b->d_tab);
-  if (b->d_pop_units) hipFree(b->d_pop_units);
-  b->d_tab = nullptr;
-  b->d_pop_units = nullptr;
-  b->lt.clear();
-  b->lp.clear();
-  b->levels.clear();
+  for (BatchPart &P : b->parts) {
+    if (P.d_tab) hipFree(P.d_tab);
+    if (P.d_pop_units) hipFree(P.d_pop_units);
+    if (P.stream) hipStreamDestroy(P.stream);
+    if (P.done) hipEventDestroy(P.done);
+  }
+  b->parts.clear();
+  if (b->d_scat_all) hipFree(b->d_scat_all);
+  b->d_scat_all = nullptr;
 }
 
 int kr_batch_destroy(kr_batch *b) {
@@ -1294,7 +1311,7 @@ extern "C" {
 static int batch_coupling(kr_batch *b, std::vector<int32_t> &coupled) {
   const size_t n = b->probs.size();
   hipStream_t st = b->ctx->stream;
-  int rc = launch_sigs_structure_multi(st, b->d_scat_prob, (int)n);
+  int rc = launch_sigs_structure_multi(st, b->d_scat_all, (int)n);
   if (rc) return rc;
   std::vector<int32_t> f;
   if ((rc = download(b->ctx, f, b->d_flags, n * SIGS_FLAG_BLOCKS))) return rc;
@@ -1304,12 +1321,12 @@ static int batch_coupling(kr_batch *b, std::vector<int32_t> &coupled) {
   return KR_OK;
 }
 
-// Builds the combined tables and the launch lists from the problems' current
-// device tables and pipelined sweep schedules.
-static int batch_build(kr_batch *b) {
-  batch_free_tables(b);
+// Builds one part's combined tables and launch lists from its problems'
+// current device tables and pipelined sweep schedules.
+static int build_part(kr_batch *b, BatchPart &B) {
   kr_context *ctx = b->ctx;
-  const int np = (int)b->probs.size();
+  const int np = B.np;
+  kr_problem *const *probs = b->probs.data() + B.p0;
   std::vector<DevSdom> sd;
   std::vector<DevPR> pr;
   std::vector<DevR> rr;
@@ -1320,9 +1337,9 @@ static int batch_build(kr_batch *b) {
   // per problem: its pipelined units (combined sdom ids, PK in `reserved`)
   std::vector<std::vector<SweepUnit>> units(np);
   int rc;
-  b->max_gz = b->max_gmix = 0;
+  B.max_gz = B.max_gmix = 0;
   for (int i = 0; i < np; ++i) {
-    kr_problem *p = b->probs[i];
+    kr_problem *p = probs[i];
     sd_off[i] = (int32_t)sd.size();
     pr_off[i] = (int32_t)pr.size();
     const int32_t r_off = (int32_t)rr.size(), s_off = (int32_t)prs.size();
@@ -1345,13 +1362,13 @@ static int batch_build(kr_batch *b) {
       P.sdom_begin += s_off;
       pr.push_back(P);
       pr_prob.push_back(i);
-      b->max_gz = std::max<int64_t>(b->max_gz, (int64_t)P.ng * P.nz);
+      B.max_gz = std::max<int64_t>(B.max_gz, (int64_t)P.ng * P.nz);
     }
     for (const DevR &R : hr) {
       rr.push_back(R);
     }
     for (size_t k = 0; k < p->h_pr.size(); ++k)
-      b->max_gmix = std::max<int64_t>(b->max_gmix, (int64_t)p->h_pr[k].ng * hr[p->h_pr[k].r].num_mix);
+      B.max_gmix = std::max<int64_t>(B.max_gmix, (int64_t)p->h_pr[k].ng * hr[p->h_pr[k].r].num_mix);
     for (int32_t s : hprs) prs.push_back(s + sd_off[i]);
     planes.insert(planes.end(), hpl.begin(), hpl.end());
     DevScat Q{};
@@ -1360,15 +1377,15 @@ static int batch_build(kr_batch *b) {
     Q.nmat = p->nmat;
     Q.L1 = p->L1;
     Q.G = p->G;
-    Q.strength = b->strength[i];
+    Q.strength = b->strength[B.p0 + i];
     Q.M = p->M;
-    Q.problem = i;
+    Q.problem = B.p0 + i;
     scat_prob.push_back(Q);  // flags pointer set after the upload
     for (SweepUnit &u : units[i]) u.sdom += sd_off[i];
   }
-  b->nsd = (int32_t)sd.size();
-  b->npr = (int32_t)pr.size();
-  for (int32_t k = 0; k < b->npr; ++k) scat_chunk.push_back(scat_prob[pr_prob[k]]);
+  B.nsd = (int32_t)sd.size();
+  B.npr = (int32_t)pr.size();
+  for (int32_t k = 0; k < B.npr; ++k) scat_chunk.push_back(scat_prob[pr_prob[k]]);
 
   // LTimes / LPlusTimes groups: chunks of equal kernel variant, reordered
   // so each group is one contiguous range
@@ -1377,14 +1394,14 @@ static int batch_build(kr_batch *b) {
     std::vector<int64_t> key(np);
     std::vector<int> dfast(np);
     for (int i = 0; i < np; ++i) {
-      LaunchCtx L = launch_ctx(b->probs[i]);
+      LaunchCtx L = launch_ctx(probs[i]);
       key[i] = ltimes_variant(L, lplus);
       dfast[i] = L.dfast;
     }
-    for (int32_t k = 0; k < b->npr; ++k) g[{key[pr_prob[k]], dfast[pr_prob[k]]}].push_back(k);
+    for (int32_t k = 0; k < B.npr; ++k) g[{key[pr_prob[k]], dfast[pr_prob[k]]}].push_back(k);
     for (auto &kv : g) {
       BatchLt G;
-      const kr_problem *p0 = b->probs[pr_prob[kv.second[0]]];
+      const kr_problem *p0 = probs[pr_prob[kv.second[0]]];
       G.M = p0->M;
       G.nd = p0->h_sd[0].nd;
       G.dfast = kv.first.second;
@@ -1398,8 +1415,8 @@ static int batch_build(kr_batch *b) {
     }
   };
   std::vector<DevPR> pr_lt, pr_lp;
-  group(false, pr_lt, b->lt);
-  group(true, pr_lp, b->lp);
+  group(false, pr_lt, B.lt);
+  group(true, pr_lp, B.lp);
 
   // sweep launches per dependency level: the row-scan kernel takes units of
   // any extents with the same row groups (RG, V) and pair order; other
@@ -1408,12 +1425,12 @@ static int batch_build(kr_batch *b) {
   std::vector<std::vector<int64_t>> pos(np);  // combined position of each problem's unit
   for (int i = 0; i < np; ++i) pos[i].assign(units[i].size(), -1);
   size_t nlev = 0;
-  for (kr_problem *p : b->probs) nlev = std::max(nlev, p->pipe_level_begin.size());
+  for (int i = 0; i < np; ++i) nlev = std::max(nlev, probs[i]->pipe_level_begin.size());
   for (size_t lv = 0; lv < nlev; ++lv) {
     std::vector<std::vector<int>> groups;
     std::vector<SweepShape> gshape;
     for (int i = 0; i < np; ++i) {
-      const kr_problem *p = b->probs[i];
+      const kr_problem *p = probs[i];
       if (lv >= p->pipe_level_begin.size()) continue;
       const SweepShape &s = p->shape_pipe;
       size_t gi = 0;
@@ -1432,36 +1449,36 @@ static int batch_build(kr_batch *b) {
     }
     std::vector<BatchSweep> launches;
     auto emit = [&](const std::vector<int> &members, SweepShape sh) {
-      BatchSweep B;
+      BatchSweep bs;
       int64_t coef = 0;
       for (int i : members) {
-        const DevSdom &S = b->probs[i]->h_sd[0];
+        const DevSdom &S = probs[i]->h_sd[0];
         coef = std::max<int64_t>(coef, (int64_t)S.nd * (S.ni + S.nj + S.nk + 1));
       }
       const int nt = sh.kind == 1 ? sh.RG * sh.PK : sh.Q * sh.PJ * sh.PK;
       const size_t lds = ((sh.kind == 1 ? 2 * (size_t)sh.V : 4) * nt + (size_t)coef) * sizeof(double);
       if (lds > ctx->lds_max || nt > 1024) return false;
-      B.shape = sh;
-      B.coef = coef;
-      B.ubeg = (int32_t)cu.size();
+      bs.shape = sh;
+      bs.coef = coef;
+      bs.ubeg = (int32_t)cu.size();
       for (int i : members) {
-        const kr_problem *p = b->probs[i];
+        const kr_problem *p = probs[i];
         const int32_t ub = p->pipe_level_begin[lv], uc = p->pipe_level_count[lv];
         for (int32_t u = ub; u < ub + uc; ++u) {
           pos[i][u] = (int64_t)cu.size();
           cu.push_back(units[i][u]);
         }
       }
-      B.ucnt = (int32_t)cu.size() - B.ubeg;
-      launches.push_back(B);
+      bs.ucnt = (int32_t)cu.size() - bs.ubeg;
+      launches.push_back(bs);
       return true;
     };
     for (size_t gi = 0; gi < groups.size(); ++gi) {
       if (emit(groups[gi], gshape[gi])) continue;
       for (int i : groups[gi])  // too large together: each problem on its own
-        if (!emit({i}, b->probs[i]->shape_pipe)) return fail(KR_ERR_UNSUPPORTED, "batch: sweep shape does not fit");
+        if (!emit({i}, probs[i]->shape_pipe)) return fail(KR_ERR_UNSUPPORTED, "batch: sweep shape does not fit");
     }
-    b->levels.push_back(launches);
+    B.levels.push_back(launches);
   }
   std::vector<int64_t> plist, pbegin{0};
   for (int i = 0; i < np; ++i) {
@@ -1474,49 +1491,100 @@ static int batch_build(kr_batch *b) {
 
   TabImage im;
   const size_t o_sd = im.add(sd), o_pr = im.add(pr), o_lt = im.add(pr_lt), o_lp = im.add(pr_lp), o_r = im.add(rr),
-               o_prs = im.add(prs), o_pl = im.add(planes), o_u = im.add(cu), o_sp = im.add(scat_prob),
+               o_prs = im.add(prs), o_pl = im.add(planes), o_u = im.add(cu),
                o_sc = im.add(scat_chunk), o_l = im.add(plist), o_b = im.add(pbegin);
-  KR_HIP_CHECK(hipMalloc((void **)&b->d_tab, std::max<size_t>(im.bytes.size(), 256)));
-  KR_HIP_CHECK(hipMalloc((void **)&b->d_pop_units, sizeof(double) * std::max<size_t>(cu.size(), 1)));
-  char *base = b->d_tab;
-  b->d_sd = (DevSdom *)(base + o_sd);
-  b->d_pr = (DevPR *)(base + o_pr);
-  b->d_pr_lt = (DevPR *)(base + o_lt);
-  b->d_pr_lp = (DevPR *)(base + o_lp);
-  b->d_r = (DevR *)(base + o_r);
-  b->d_pr_sdoms = (int32_t *)(base + o_prs);
-  b->d_planes = (SweepPlanes *)(base + o_pl);
-  b->d_units = (SweepUnit *)(base + o_u);
-  b->d_scat_prob = (DevScat *)(base + o_sp);
-  b->d_scat_chunk = (DevScat *)(base + o_sc);
-  b->d_pop_list = (int64_t *)(base + o_l);
-  b->d_pop_begin = (int64_t *)(base + o_b);
-  // flags: each problem's share of the batch's flag array
-  for (int i = 0; i < np; ++i) {
-    DevScat *q = (DevScat *)(im.bytes.data() + o_sp) + i;
-    q->flags = b->d_flags + (int64_t)i * SIGS_FLAG_BLOCKS;
-  }
-  for (int32_t k = 0; k < b->npr; ++k) {
+  KR_HIP_CHECK(hipMalloc((void **)&B.d_tab, std::max<size_t>(im.bytes.size(), 256)));
+  KR_HIP_CHECK(hipMalloc((void **)&B.d_pop_units, sizeof(double) * std::max<size_t>(cu.size(), 1)));
+  char *base = B.d_tab;
+  B.d_sd = (DevSdom *)(base + o_sd);
+  B.d_pr = (DevPR *)(base + o_pr);
+  B.d_pr_lt = (DevPR *)(base + o_lt);
+  B.d_pr_lp = (DevPR *)(base + o_lp);
+  B.d_r = (DevR *)(base + o_r);
+  B.d_pr_sdoms = (int32_t *)(base + o_prs);
+  B.d_planes = (SweepPlanes *)(base + o_pl);
+  B.d_units = (SweepUnit *)(base + o_u);
+  B.d_scat_chunk = (DevScat *)(base + o_sc);
+  B.d_pop_list = (int64_t *)(base + o_l);
+  B.d_pop_begin = (int64_t *)(base + o_b);
+  // flags: each chunk's problem's share of the batch's flag array
+  for (int32_t k = 0; k < B.npr; ++k) {
     DevScat *q = (DevScat *)(im.bytes.data() + o_sc) + k;
-    q->flags = b->d_flags + (int64_t)pr_prob[k] * SIGS_FLAG_BLOCKS;
+    q->flags = b->d_flags + (int64_t)(B.p0 + pr_prob[k]) * SIGS_FLAG_BLOCKS;
   }
-  KR_HIP_CHECK(hipMemcpyAsync(b->d_tab, im.bytes.data(), im.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
+  KR_HIP_CHECK(hipMemcpyAsync(B.d_tab, im.bytes.data(), im.bytes.size(), hipMemcpyHostToDevice, ctx->stream));
   KR_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   return KR_OK;
 }
 
-// One source iteration of every problem (the mode-1 sequence of the host
-// mirror's SteadyStateIteration), population into out[problem*stride].
+// Splits the problems into consecutive parts of about equal work (unknowns),
+// each advanced on its own graph branch, so that the stages of different
+// parts overlap (one part's latency-bound sweeps beside another's streaming
+// LTimes).  Builds every part and the per-problem structure-check table.
+static int batch_build(kr_batch *b) {
+  batch_free_tables(b);
+  const int np = (int)b->probs.size();
+  std::vector<double> work(np, 0.0);
+  double total = 0.0;
+  for (int i = 0; i < np; ++i) {
+    for (const DevSdom &S : b->probs[i]->h_sd) work[i] += (double)S.nd * S.ng * S.nz;
+    total += work[i];
+  }
+  const int nparts = std::max(1, std::min(BATCH_PARTS, np));
+  double acc = 0.0;
+  int start = 0;
+  for (int k = 0; k < nparts && start < np; ++k) {
+    int end = start + 1;  // at least one problem, and leave one for each later part
+    acc += work[start];
+    const double goal = total * (k + 1) / nparts;
+    while (end < np - (nparts - k - 1) && acc + 0.5 * work[end] <= goal) acc += work[end++];
+    if (k == nparts - 1)
+      while (end < np) acc += work[end++];
+    BatchPart part;
+    part.p0 = start;
+    part.np = end - start;
+    b->parts.push_back(part);
+    start = end;
+  }
+  for (BatchPart &part : b->parts) {
+    int rc = build_part(b, part);
+    if (rc) return rc;
+    if (b->parts.size() > 1) {
+      KR_HIP_CHECK(hipStreamCreateWithFlags(&part.stream, hipStreamNonBlocking));
+      KR_HIP_CHECK(hipEventCreateWithFlags(&part.done, hipEventDisableTiming));
+    }
+  }
+  std::vector<DevScat> all(np);
+  for (int i = 0; i < np; ++i) {
+    const kr_problem *p = b->probs[i];
+    DevScat &Q = all[i];
+    Q.sigs = p->sigs;
+    for (int k = 0; k < 4; ++k) Q.str[k] = p->sigs_str[k];
+    Q.nmat = p->nmat;
+    Q.L1 = p->L1;
+    Q.G = p->G;
+    Q.flags = b->d_flags + (int64_t)i * SIGS_FLAG_BLOCKS;
+    Q.strength = b->strength[i];
+    Q.M = p->M;
+    Q.problem = i;
+  }
+  return upload(b->ctx, &b->d_scat_all, all);
+}
+
+// One source iteration of every problem of a part (the mode-1 sequence of
+// the host mirror's SteadyStateIteration), population of its problem i into
+// out[i*stride].  branches: the launches of one stage on graph branches.
 // Only kernel launches: capturable.
-static int batch_iteration(kr_batch *b, hipStream_t st, double *out, int64_t stride) {
+static int batch_iteration(kr_batch *b, const BatchPart &B, hipStream_t st, double *out, int64_t stride,
+                           bool branches) {
   int rc;
   auto ctx_of = [&](const BatchLt &G, DevPR *tab) {
     LaunchCtx L{};
     L.stream = st;
-    L.sd = b->d_sd;
+    L.sd = B.d_sd;
     L.pr = tab + G.pbeg;
-    L.r = b->d_r;
-    L.pr_sdoms = b->d_pr_sdoms;
+    L.r = B.d_r;
+    L.pr_sdoms = B.d_pr_sdoms;
     L.num_pr = G.pcnt;
     L.num_moments = G.M;
     L.nd = G.nd;
@@ -1525,37 +1593,37 @@ static int batch_iteration(kr_batch *b, hipStream_t st, double *out, int64_t str
     L.max_gz = G.max_gz;
     return L;
   };
-  if ((rc = batch_parallel(b, st, (int)b->lt.size(), [&](hipStream_t s, int i) {
-         LaunchCtx L = ctx_of(b->lt[i], b->d_pr_lt);
+  if ((rc = batch_parallel(b, st, (int)B.lt.size(), branches, [&](hipStream_t s, int i) {
+         LaunchCtx L = ctx_of(B.lt[i], B.d_pr_lt);
          L.stream = s;
          return launch_ltimes(L, 0);
        })))
     return rc;
-  if ((rc = launch_scatter_diag_multi(st, b->d_pr, b->d_r, b->d_scat_chunk, b->npr, b->max_gz))) return rc;
-  for (size_t i = 0; i < b->probs.size(); ++i) {
+  if ((rc = launch_scatter_diag_multi(st, B.d_pr, B.d_r, B.d_scat_chunk, B.npr, B.max_gz))) return rc;
+  for (int i = B.p0; i < B.p0 + B.np; ++i) {
     if (!b->coupled[i]) continue;
     LaunchCtx L = launch_ctx(b->probs[i]);
     L.stream = st;
     if ((rc = launch_scattering(L, b->probs[i]->pr_gset.data(), b->probs[i]->pr_r.data(), 0))) return rc;
   }
-  if ((rc = launch_source_multi(st, b->d_pr, b->d_r, b->d_scat_chunk, b->npr, b->max_gmix))) return rc;
-  if ((rc = batch_parallel(b, st, (int)b->lp.size(), [&](hipStream_t s, int i) {
-         LaunchCtx L = ctx_of(b->lp[i], b->d_pr_lp);
+  if ((rc = launch_source_multi(st, B.d_pr, B.d_r, B.d_scat_chunk, B.npr, B.max_gmix))) return rc;
+  if ((rc = batch_parallel(b, st, (int)B.lp.size(), branches, [&](hipStream_t s, int i) {
+         LaunchCtx L = ctx_of(B.lp[i], B.d_pr_lp);
          L.stream = s;
          return launch_lplustimes(L, 0);
        })))
     return rc;
-  if ((rc = launch_sweep_coef(st, b->d_sd, 0, b->nsd))) return rc;
-  for (const auto &lv : b->levels)
-    if ((rc = batch_parallel(b, st, (int)lv.size(), [&](hipStream_t s, int i) {
-           const BatchSweep &B = lv[i];
-           if (B.shape.kind == 1)
-             return launch_sweep_rows(s, b->d_sd, b->d_planes, b->d_units + B.ubeg, B.ucnt, B.shape, B.coef,
-                                      b->d_pop_units + B.ubeg, false);
-           return launch_sweep(s, b->d_sd, b->d_planes, b->d_units + B.ubeg, B.ucnt, B.shape, b->d_pop_units + B.ubeg);
+  if ((rc = launch_sweep_coef(st, B.d_sd, 0, B.nsd))) return rc;
+  for (const auto &lv : B.levels)
+    if ((rc = batch_parallel(b, st, (int)lv.size(), branches, [&](hipStream_t s, int i) {
+           const BatchSweep &W = lv[i];
+           if (W.shape.kind == 1)
+             return launch_sweep_rows(s, B.d_sd, B.d_planes, B.d_units + W.ubeg, W.ucnt, W.shape, W.coef,
+                                      B.d_pop_units + W.ubeg, false);
+           return launch_sweep(s, B.d_sd, B.d_planes, B.d_units + W.ubeg, W.ucnt, W.shape, B.d_pop_units + W.ubeg);
          })))
       return rc;
-  return launch_pop_sum_multi(st, b->d_pop_units, b->d_pop_list, b->d_pop_begin, (int)b->probs.size(), out, stride);
+  return launch_pop_sum_multi(st, B.d_pop_units, B.d_pop_list, B.d_pop_begin, B.np, out, stride);
 }
 
 // Builds the tables and the graph of niter iterations.
@@ -1580,7 +1648,24 @@ static int batch_capture(kr_batch *b, int32_t niter, const std::vector<int32_t> 
   KR_HIP_CHECK(hipStreamSynchronize(s0));
   KR_HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
   int rc = KR_OK;
-  for (int32_t it = 0; !rc && it < niter; ++it) rc = batch_iteration(b, s0, b->d_pop + it, niter);
+  if (b->parts.size() == 1) {
+    for (int32_t it = 0; !rc && it < niter; ++it) rc = batch_iteration(b, b->parts[0], s0, b->d_pop + it, niter, true);
+  } else {
+    // each part on its own branch, its stages on that branch in order
+    if (hipEventRecord(b->fork, s0) != hipSuccess) rc = fail(KR_ERR_HIP, "batch capture: fork");
+    for (const BatchPart &part : b->parts) {
+      if (rc) break;
+      if (hipStreamWaitEvent(part.stream, b->fork, 0) != hipSuccess) {
+        rc = fail(KR_ERR_HIP, "batch capture: branch");
+        break;
+      }
+      for (int32_t it = 0; !rc && it < niter; ++it)
+        rc = batch_iteration(b, part, part.stream, b->d_pop + (int64_t)part.p0 * niter + it, niter, false);
+      if (!rc && (hipEventRecord(part.done, part.stream) != hipSuccess ||
+                  hipStreamWaitEvent(s0, part.done, 0) != hipSuccess))
+        rc = fail(KR_ERR_HIP, "batch capture: join");
+    }
+  }
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s0, &g);
   if (rc) {
