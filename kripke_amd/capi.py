@@ -3,6 +3,7 @@
 Loads the in-tree library and fails loudly when it is missing: there is no
 CPU fallback behind this module.
 """
+import atexit
 import ctypes
 import weakref
 import os
@@ -178,6 +179,21 @@ def autotune_nesting(ctx, inp):
     return LAYOUTS[nest.value], ms.value
 
 
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    """Contexts still open at interpreter exit are closed (problems first)
+    while the HIP runtime is still up, not from objects' finalizers in
+    whatever order the interpreter tears modules down."""
+    for ctx in list(_live_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """kr_context: one device, one HIP stream."""
 
@@ -185,6 +201,7 @@ class Context:
         self.h = P()
         self._children = weakref.WeakSet()  # problems built on this context
         check(lib().kr_context_create(device, ctypes.byref(self.h)), "kr_context_create")
+        _live_contexts.add(self)
 
     def synchronize(self):
         check(lib().kr_synchronize(self.h), "kr_synchronize")
