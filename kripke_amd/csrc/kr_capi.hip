@@ -80,6 +80,7 @@ struct kr_problem {
   int64_t sigs_str[4] = {0, 0, 0, 0};
   int32_t *d_sigs_flags = nullptr;
   double *d_sigs_t = nullptr;                 // dense scattering: sigs re-laid output-group fastest
+  double *d_vol_info = nullptr;               // per R chunk: {uniform volume flag, volume} (k_vol_info)
   int64_t max_gz = 0, max_gmix = 0, max_g = 0, max_z = 0;
   // timing
   int timing = 0;
@@ -415,6 +416,11 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     kr_problem_destroy(p);
     return fail(KR_ERR_INVALID, "kr_problem_create: " + m);
   };
+  if (p->nr > 0 && hipMalloc((void **)&p->d_vol_info, sizeof(double) * 2 * p->nr) != hipSuccess) {
+    p->d_vol_info = nullptr;
+    kr_problem_destroy(p);
+    return fail(KR_ERR_NOMEM, "kr_problem_create: volume table");
+  }
 
   // --- validate subdomains ------------------------------------------------
   const kr_sdom_desc &s0 = p->sdoms[0];
@@ -488,6 +494,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     H.ell = S.ell;
     H.ell_plus = S.ell_plus;
     H.volume = p->rs[S.r_chunk].volume;
+    H.vol_info = p->d_vol_info + 2 * S.r_chunk;
     H.nd = nd; H.ng = ng; H.ni = ni; H.nj = nj; H.nk = nk; H.nz = (int32_t)nz;
     H.id = sgn[0]; H.jd = sgn[1]; H.kd = sgn[2];
     H.pr = S.pr_chunk; H.r = S.r_chunk; H.gset = S.gset; H.dset = S.dset; H.glo = S.group_lower;
@@ -550,6 +557,7 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
     H.mix_mat = R.mixelem_to_material;
     H.mix_frac = R.mixelem_to_fraction;
     H.volume = R.volume;
+    H.vol_info = p->d_vol_info + 2 * c;
     H.num_mix = R.num_mixelem;
     H.nz = (int32_t)nz;
     p->max_gmix = std::max<int64_t>(p->max_gmix, (int64_t)ng * R.num_mixelem);
@@ -671,7 +679,8 @@ int kr_problem_destroy(kr_problem *p) {
   void *bufs[] = {p->d_sd,          p->d_pr,        p->d_r,          p->d_pr_sdoms,  p->d_coef,
                   p->d_planes_ref,  p->d_planes_pipe, p->d_units_ref, p->d_units_pipe, p->d_units_tmp,
                   p->d_pop_blocks,  p->d_pop_partial, p->d_pop_out,  p->d_sigs_flags, p->d_pop_units,
-                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t, p->d_planes_bj, p->d_planes_adj_seed};
+                  p->d_planes_adj,  (void *)p->d_ptrs, p->d_sigs_t, p->d_planes_bj, p->d_planes_adj_seed,
+                  p->d_vol_info};
   for (void *b : bufs)
     if (b) hipFree(b);
   delete p;
@@ -745,6 +754,8 @@ static int sweep_solver_impl(kr_problem *p, const int32_t *list, int32_t num, do
       pop = p->d_pop_units;
       *pop_units = pop;
       *npop = n;
+      int rc = launch_vol_info(p->ctx->stream, p->d_r, p->nr);
+      if (rc) return rc;
     }
     for (size_t lv = 0; lv < p->pipe_level_begin.size(); ++lv) {
       CallTimer tk(p, "k_sweep");  // one sweep-kernel launch (one zone-set stage)
@@ -1167,7 +1178,7 @@ struct BatchPart {
   DevScat *d_scat_chunk = nullptr;
   int64_t *d_pop_list = nullptr, *d_pop_begin = nullptr;
   double *d_pop_units = nullptr;     // one population partial per combined unit
-  int32_t nsd = 0, npr = 0;
+  int32_t nsd = 0, npr = 0, nr = 0;
   int64_t max_gz = 0, max_gmix = 0;
   std::vector<BatchLt> lt, lp;
   std::vector<std::vector<BatchSweep>> levels;
@@ -1384,6 +1395,7 @@ static int build_part(kr_batch *b, BatchPart &B) {
     for (SweepUnit &u : units[i]) u.sdom += sd_off[i];
   }
   B.nsd = (int32_t)sd.size();
+  B.nr = (int32_t)rr.size();
   B.npr = (int32_t)pr.size();
   for (int32_t k = 0; k < B.npr; ++k) scat_chunk.push_back(scat_prob[pr_prob[k]]);
 
@@ -1613,7 +1625,7 @@ static int batch_iteration(kr_batch *b, const BatchPart &B, hipStream_t st, doub
          return launch_lplustimes(L, 0);
        })))
     return rc;
-  if ((rc = launch_sweep_coef(st, B.d_sd, 0, B.nsd))) return rc;
+  if ((rc = launch_sweep_coef(st, B.d_sd, 0, B.nsd)) || (rc = launch_vol_info(st, B.d_r, B.nr))) return rc;
   for (const auto &lv : B.levels)
     if ((rc = batch_parallel(b, st, (int)lv.size(), branches, [&](hipStream_t s, int i) {
            const BatchSweep &W = lv[i];

@@ -74,6 +74,7 @@ struct DevSdom {
   const double *delta[3];     // dx, dy, dz chunks
   const double *w, *ell, *ell_plus;
   const double *volume;
+  const double *vol_info;     // {1 if every zone volume of the R chunk is the same, that volume} (k_vol_info)
   int32_t nd, ng, ni, nj, nk, nz;
   int32_t id, jd, kd;
   int32_t pr, r, gset, dset, glo;
@@ -99,6 +100,7 @@ struct DevR {
   const int64_t *mix_zone, *mix_mat;
   const double *mix_frac;
   const double *volume;
+  double *vol_info;           // written by k_vol_info: {uniform flag, first volume}
   int64_t num_mix;
   int32_t nz;
 };

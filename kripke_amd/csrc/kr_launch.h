@@ -95,6 +95,8 @@ int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *plan
                       int32_t num_units, const SweepShape &shape, int64_t coef_doubles, double *pop, bool adjoint);
 // Lets `kernel` take `bytes` of dynamic LDS (needed above 64 KiB).
 int allow_lds(const void *kernel, size_t bytes);
+// vol_info of R chunks [0, nr): read by the row-scan sweep's fused population.
+int launch_vol_info(hipStream_t st, const DevR *r, int nr);
 // Sum of n doubles in a fixed order (one workgroup): out[0] = sum(partial[0..n)).
 int launch_sum(hipStream_t st, const double *partial, int64_t n, double *out);
 // Refresh the 2*cos/delta tables of subdomains [first, first+count).

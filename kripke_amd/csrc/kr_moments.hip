@@ -845,6 +845,27 @@ __global__ __launch_bounds__(256) void k_pop_sum_multi(const double *__restrict_
   if (threadIdx.x == 0) out[(int64_t)blockIdx.x * out_stride] = red[0];
 }
 
+// Per R chunk (block b): whether every zone volume equals the first (bitwise
+// ==, so NaN never counts as uniform), and that volume.  The fused population
+// of the row-scan sweep then takes the volume from this pair instead of
+// loading it per zone (the reference's generator makes it one value,
+// src/Kripke/Generate/Space.cpp:133-136).
+__global__ __launch_bounds__(256) void k_vol_info(const DevR *__restrict__ rs) {
+  const DevR &R = rs[blockIdx.x];
+  __shared__ int differs;
+  if (threadIdx.x == 0) differs = 0;
+  __syncthreads();
+  const double v0 = R.volume[0];
+  int mine = !(v0 == v0);
+  for (int64_t z = threadIdx.x; z < R.nz && !mine; z += blockDim.x) mine = !(R.volume[z] == v0);
+  if (mine) atomicOr(&differs, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R.vol_info[0] = differs ? 0.0 : 1.0;
+    R.vol_info[1] = v0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Population: src/Kripke/Kernel/Population.cpp:49-63.  Term order per
 // element is (w*psi)*volume; the sum is a fixed two-level tree, so the
@@ -1155,6 +1176,13 @@ int launch_population(const LaunchCtx &L, const void *blocks, int64_t num_blocks
                      (const PopBlock *)blocks, partial);
   KR_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(k_pop_final, dim3(1), dim3(256), 0, L.stream, partial, num_blocks, out);
+  KR_HIP_CHECK(hipGetLastError());
+  return KR_OK;
+}
+
+int launch_vol_info(hipStream_t st, const DevR *r, int nr) {
+  if (nr <= 0) return KR_OK;
+  hipLaunchKernelGGL(k_vol_info, dim3(nr), dim3(256), 0, st, r);
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }

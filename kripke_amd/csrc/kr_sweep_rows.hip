@@ -165,6 +165,8 @@ struct RowArgs {
   const double *in[3];
   double *out[3];
   const double *w, *vol;  // quadrature weights, zone volumes (population)
+  double vol0;            // the zone volume when all are equal (vol_uniform)
+  int32_t vol_uniform;
   double *own2;           // the subdomain's own k-plane: faces between k-patches
   double *pop;            // population partial of this block, or nullptr
   int32_t nd, ng, ni, nj, nk, jd, kd;
@@ -389,7 +391,16 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
     cursor_step(lc, P, K);
   };
 
-  if (POP) load_v<V, 0>(rv, Rvol, cc.vo8);
+  // volumes: one value for the whole subdomain, or one L2 load per row
+  auto vol_row = [&]() {
+    if (P.vol_uniform) {
+#pragma unroll
+      for (int e = 0; e < V; ++e) rv[e] = P.vol0;
+    } else {
+      load_v<V, 0>(rv, Rvol, cc.vo8);
+    }
+  };
+  if (POP) vol_row();
 #pragma unroll
   for (int r = 0; r < RING; ++r) refill(r);
 
@@ -498,7 +509,7 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
 #pragma unroll
       for (int m = 0; m < V; ++m) fzb[(cur * nt + tid) * V + m] = fzo[memo<SX, V>(m)];
       cursor_step(cc, P, K);
-      if (POP) load_v<V, 0>(rv, Rvol, cc.vo8);
+      if (POP) vol_row();
       refill(r);
       __syncthreads();
     }
@@ -552,6 +563,8 @@ __global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__
   P.inner_is_g = sh.inner_is_g;
   P.w = S.w;
   P.vol = S.volume;
+  P.vol_uniform = POP && S.vol_info[0] != 0.0;
+  P.vol0 = POP ? S.vol_info[1] : 0.0;
   P.own2 = S.plane[2];
   P.pop = POP ? pop + blockIdx.x : nullptr;
   if ((S.id > 0) != ADJ)
