@@ -35,15 +35,14 @@ CPU_SAMPLE = dict(zones=(32, 32, 32), groups=64, quad=(8, 16), legendre=4, dset=
                   geometry=1, seed=20261202)
 
 
-def algorithmic_bytes(nd_total, ng, nz, moments, num_sdoms_per_zoneset):
+def algorithmic_bytes(nd_total, ng, nz, moments):
     """Compulsory HBM bytes per iteration of the metric path at the kept
     entry-point boundary: LTimes reads psi (8N) and writes phi (8P),
     LPlusTimes reads phi_out (8P) and writes rhs (8N), the sweep reads rhs
-    and writes psi (16N) and reads sigma_t once per subdomain (8GZ each)."""
+    and writes psi (16N) and reads sigma_t once (8GZ): 32N + 16P + 8GZ."""
     N = nd_total * ng * nz
     P = moments * ng * nz
-    return {"ltimes": 8 * N + 8 * P, "lplustimes": 8 * P + 8 * N,
-            "sweep": 16 * N + 8 * ng * nz * num_sdoms_per_zoneset, "N": N, "P": P}
+    return {"ltimes": 8 * N + 8 * P, "lplustimes": 8 * P + 8 * N, "sweep": 16 * N + 8 * ng * nz, "N": N, "P": P}
 
 
 def measured_traffic(layout):
@@ -367,7 +366,7 @@ def main():
     elapsed, setup_s, timers, nb = run_gpu(args, args.layout)
     nz = 64 ** 3
     nd, ng, M = 128, 128, 25
-    ab = algorithmic_bytes(nd, ng, nz, M, num_sdoms_per_zoneset=8)
+    ab = algorithmic_bytes(nd, ng, nz, M)
     N = ab["N"]
     ms = elapsed / args.steps * 1e3
     value = N * args.steps / elapsed

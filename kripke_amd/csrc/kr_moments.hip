@@ -850,7 +850,7 @@ __global__ __launch_bounds__(256) void k_pop_sum_multi(const double *__restrict_
 // of the row-scan sweep then takes the volume from this pair instead of
 // loading it per zone (the reference's generator makes it one value,
 // src/Kripke/Generate/Space.cpp:133-136).
-__global__ __launch_bounds__(256) void k_vol_info(const DevR *__restrict__ rs) {
+__global__ __launch_bounds__(1024) void k_vol_info(const DevR *__restrict__ rs) {
   const DevR &R = rs[blockIdx.x];
   __shared__ int differs;
   if (threadIdx.x == 0) differs = 0;
@@ -1182,7 +1182,7 @@ int launch_population(const LaunchCtx &L, const void *blocks, int64_t num_blocks
 
 int launch_vol_info(hipStream_t st, const DevR *r, int nr) {
   if (nr <= 0) return KR_OK;
-  hipLaunchKernelGGL(k_vol_info, dim3(nr), dim3(256), 0, st, r);
+  hipLaunchKernelGGL(k_vol_info, dim3(nr), dim3(1024), 0, st, r);
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }
