@@ -81,7 +81,9 @@ __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, 
     for (int d0 = 0; d0 < nd; d0 += DB) ltimes_batch<M, DB>(acc, psi + d0 * fa, fa, ell + d0 * M);  // DB divides nd
   }
 #pragma unroll
-  for (int nm = 0; nm < M; ++nm) phi[po + nm * P.ma] = acc[nm];
+  // streaming (non-temporal) stores: phi is not re-read by this kernel, and
+  // keeping its M planes out of the caches leaves them to the psi stream
+  for (int nm = 0; nm < M; ++nm) __builtin_nontemporal_store(acc[nm], phi + po + nm * P.ma);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,10 +341,10 @@ __global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ 
     const double *src = ADJ ? P.phi_out : P.phi;
     const double base0 = accumulate ? dst[po] : 0.0;
     const double t = s * src[po];
-    dst[po] = base0 + t;
+    __builtin_nontemporal_store(base0 + t, dst + po);
     for (int nm = 1; nm < M; ++nm) {
       const double b = accumulate ? dst[po + nm * P.ma] : 0.0;
-      dst[po + nm * P.ma] = b + 0.0;
+      __builtin_nontemporal_store(b + 0.0, dst + po + nm * P.ma);
     }
   }
 }
@@ -670,8 +672,9 @@ __global__ __launch_bounds__(256) void k_scatter_zonal(const DevPR *__restrict__
     gz_of(e, P.ng, P.nz, zfast, g, z);
     const int64_t po = g * P.mg + z * P.mz;
     const double t = sc[g * s_g + z * s_z] * src[po];
-    dst[po] = (accumulate ? dst[po] : 0.0) + t;
-    for (int nm = 1; nm < M; ++nm) dst[po + nm * P.ma] = (accumulate ? dst[po + nm * P.ma] : 0.0) + 0.0;
+    __builtin_nontemporal_store((accumulate ? dst[po] : 0.0) + t, dst + po);
+    for (int nm = 1; nm < M; ++nm)
+      __builtin_nontemporal_store((accumulate ? dst[po + nm * P.ma] : 0.0) + 0.0, dst + po + nm * P.ma);
   }
 }
 
@@ -802,8 +805,8 @@ __global__ __launch_bounds__(256) void k_scatter_diag_multi(const DevPR *__restr
     const int64_t gg = P.glo + g;
     const double sz = sigs_zone(Q.sigs, ss, R, z, 0, gg, gg);
     const int64_t po = g * P.mg + z * P.mz;
-    P.phi_out[po] = 0.0 + sz * P.phi[po];
-    for (int nm = 1; nm < Q.M; ++nm) P.phi_out[po + nm * P.ma] = 0.0 + 0.0;
+    __builtin_nontemporal_store(0.0 + sz * P.phi[po], P.phi_out + po);
+    for (int nm = 1; nm < Q.M; ++nm) __builtin_nontemporal_store(0.0 + 0.0, P.phi_out + po + nm * P.ma);
   }
 }
 
