@@ -80,9 +80,9 @@ __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, 
     const cmem<double> *ell = cp(ADJ ? S.ell_plus : S.ell);
     for (int d0 = 0; d0 < nd; d0 += DB) ltimes_batch<M, DB>(acc, psi + d0 * fa, fa, ell + d0 * M);  // DB divides nd
   }
-#pragma unroll
   // streaming (non-temporal) stores: phi is not re-read by this kernel, and
   // keeping its M planes out of the caches leaves them to the psi stream
+#pragma unroll
   for (int nm = 0; nm < M; ++nm) __builtin_nontemporal_store(acc[nm], phi + po + nm * P.ma);
 }
 
