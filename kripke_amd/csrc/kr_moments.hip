@@ -35,25 +35,12 @@ __device__ inline void gz_of(int64_t e, int32_t ng, int32_t nz, bool zfast, int3
 
 // ---------------------------------------------------------------------------
 // LTimes: src/Kripke/Kernel/LTimes.cpp:54-65 (body), :79-105 (sdom loop).
-// A thread owns one (g,z) column: it issues the psi loads of a batch of DB
-// directions, then folds them into M accumulators, direction by direction,
-// with ell read through the scalar cache (SGPR operands of the FMAs).
+// A thread owns one (g,z) column and folds its psi into M accumulators in
+// batches of DB directions, direction by direction, with ell read through
+// the scalar cache (SGPR operands of the FMAs).  The loads of the next batch
+// are issued before the FMAs of the current one (a two-batch software
+// pipeline), so DB to 2 DB loads per thread are always in flight.
 // ---------------------------------------------------------------------------
-// DB directions of one column: all psi loads first, then the FMAs.
-template <int M, int DB>
-__device__ __forceinline__ void ltimes_batch(double (&acc)[M], const gmem<double> *psi, int64_t fa,
-                                             const cmem<double> *ell) {
-  double p[DB];
-#pragma unroll
-  for (int b = 0; b < DB; ++b) p[b] = __builtin_nontemporal_load(psi + b * fa);
-  __builtin_amdgcn_sched_barrier(0);  // keep all DB loads in flight before the first FMA
-#pragma unroll
-  for (int b = 0; b < DB; ++b) {
-#pragma unroll
-    for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[b * M + nm], p[b], acc[nm]);
-  }
-}
-
 // ADJ: the same contraction as LPlusTimes' transpose: phi_out(nm) (+)= sum_d
 // ell_plus(d,nm) rhs(d) (the adjoint of LPlusTimes maps rhs to phi_out).
 template <int M, int DB, bool ACC, bool ADJ>
@@ -74,11 +61,35 @@ __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, 
   double acc[M];
 #pragma unroll
   for (int nm = 0; nm < M; ++nm) acc[nm] = ACC ? phi[po + nm * P.ma] : 0.0;
-  for (int k = 0; k < P.sdom_count; ++k) {
-    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    const gmem<double> *psi = gp(ADJ ? S.rhs : S.psi) + g * S0.fg + z * S0.fz;
-    const cmem<double> *ell = cp(ADJ ? S.ell_plus : S.ell);
-    for (int d0 = 0; d0 < nd; d0 += DB) ltimes_batch<M, DB>(acc, psi + d0 * fa, fa, ell + d0 * M);  // DB divides nd
+  // software pipeline over the batches (subdomain k, directions d0..d0+DB):
+  // the next batch's loads are in flight while this batch's FMAs run
+  const int nb = nd / DB, total = P.sdom_count * nb;  // DB divides nd
+  auto batch_ptr = [&](int t, const cmem<double> *&ell) {
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + t / nb]];
+    ell = cp(ADJ ? S.ell_plus : S.ell) + (t % nb) * DB * M;
+    return gp(ADJ ? S.rhs : S.psi) + g * S0.fg + z * S0.fz + (int64_t)(t % nb) * DB * fa;
+  };
+  double cur[DB], nxt[DB];
+  const cmem<double> *ell_cur, *ell_nxt = nullptr;
+  {
+    const gmem<double> *src = batch_ptr(0, ell_cur);
+#pragma unroll
+    for (int b = 0; b < DB; ++b) cur[b] = __builtin_nontemporal_load(src + b * fa);
+  }
+  for (int t = 0; t < total; ++t) {
+    if (t + 1 < total) {
+      const gmem<double> *src = batch_ptr(t + 1, ell_nxt);
+#pragma unroll
+      for (int b = 0; b < DB; ++b) nxt[b] = __builtin_nontemporal_load(src + b * fa);
+    }
+#pragma unroll
+    for (int b = 0; b < DB; ++b) {
+#pragma unroll
+      for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell_cur[b * M + nm], cur[b], acc[nm]);
+    }
+#pragma unroll
+    for (int b = 0; b < DB; ++b) cur[b] = nxt[b];
+    ell_cur = ell_nxt;
   }
   // streaming (non-temporal) stores: phi is not re-read by this kernel, and
   // keeping its M planes out of the caches leaves them to the psi stream
