@@ -91,10 +91,18 @@ __global__ __launch_bounds__(256) void k_ltimes(const DevSdom *__restrict__ sd, 
     for (int b = 0; b < DB; ++b) cur[b] = nxt[b];
     ell_cur = ell_nxt;
   }
+  // moment planes (ma > 1: the zone- and group-fastest nestings) take
   // streaming (non-temporal) stores: phi is not re-read by this kernel, and
-  // keeping its M planes out of the caches leaves them to the psi stream
+  // keeping its M planes out of the caches leaves them to the psi stream.
+  // Moments adjacent in memory (ma == 1) keep ordinary stores, which the
+  // caches merge into whole lines.
+  if (P.ma > 1) {
 #pragma unroll
-  for (int nm = 0; nm < M; ++nm) __builtin_nontemporal_store(acc[nm], phi + po + nm * P.ma);
+    for (int nm = 0; nm < M; ++nm) __builtin_nontemporal_store(acc[nm], phi + po + nm * P.ma);
+  } else {
+#pragma unroll
+    for (int nm = 0; nm < M; ++nm) phi[po + nm] = acc[nm];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -352,10 +360,10 @@ __global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ 
     const double *src = ADJ ? P.phi_out : P.phi;
     const double base0 = accumulate ? dst[po] : 0.0;
     const double t = s * src[po];
-    __builtin_nontemporal_store(base0 + t, dst + po);
+    dst[po] = base0 + t;
     for (int nm = 1; nm < M; ++nm) {
       const double b = accumulate ? dst[po + nm * P.ma] : 0.0;
-      __builtin_nontemporal_store(b + 0.0, dst + po + nm * P.ma);
+      dst[po + nm * P.ma] = b + 0.0;
     }
   }
 }
@@ -683,9 +691,8 @@ __global__ __launch_bounds__(256) void k_scatter_zonal(const DevPR *__restrict__
     gz_of(e, P.ng, P.nz, zfast, g, z);
     const int64_t po = g * P.mg + z * P.mz;
     const double t = sc[g * s_g + z * s_z] * src[po];
-    __builtin_nontemporal_store((accumulate ? dst[po] : 0.0) + t, dst + po);
-    for (int nm = 1; nm < M; ++nm)
-      __builtin_nontemporal_store((accumulate ? dst[po + nm * P.ma] : 0.0) + 0.0, dst + po + nm * P.ma);
+    dst[po] = (accumulate ? dst[po] : 0.0) + t;
+    for (int nm = 1; nm < M; ++nm) dst[po + nm * P.ma] = (accumulate ? dst[po + nm * P.ma] : 0.0) + 0.0;
   }
 }
 
@@ -816,8 +823,8 @@ __global__ __launch_bounds__(256) void k_scatter_diag_multi(const DevPR *__restr
     const int64_t gg = P.glo + g;
     const double sz = sigs_zone(Q.sigs, ss, R, z, 0, gg, gg);
     const int64_t po = g * P.mg + z * P.mz;
-    __builtin_nontemporal_store(0.0 + sz * P.phi[po], P.phi_out + po);
-    for (int nm = 1; nm < Q.M; ++nm) __builtin_nontemporal_store(0.0 + 0.0, P.phi_out + po + nm * P.ma);
+    P.phi_out[po] = 0.0 + sz * P.phi[po];
+    for (int nm = 1; nm < Q.M; ++nm) P.phi_out[po + nm * P.ma] = 0.0 + 0.0;
   }
 }
 
