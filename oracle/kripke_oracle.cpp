@@ -648,6 +648,7 @@ extern "C" int64_t ko_set(ko_problem *p, const char *name, int32_t chunk, const 
   KO_FIELDS(X)
 #undef X
   if (n == "sigs") return copy_in(p->sigs, src);
+  if (n == "volume" && chunk >= 0 && chunk < p->nr) return copy_in(p->rd[chunk].volume, src);
   return -1;
 }
 

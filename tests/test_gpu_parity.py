@@ -340,6 +340,32 @@ def test_sweep_solver_population(ctx, layout, zones):
         k.sweep_solver_population(sdoms[:-1])
 
 
+@pytest.mark.parametrize("layout", ["DGZ", "DZG"])
+@pytest.mark.parametrize("uniform", [True, False])
+def test_sweep_solver_population_volumes(ctx, layout, uniform):
+    """The fused population with zone volumes that are all equal (the
+    generator's, taken as one value per chunk) and that differ (read per
+    zone): both against kr_population and the oracle on the same volumes,
+    and again after the volumes change between calls."""
+    kw = kwargs_for(layout, zset=(2, 1, 1), groups=8, zones=(8, 6, 10))
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(29)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    k = hp.kernels()
+    sdoms = list(range(hp.num_sdoms))
+    for trial in range(2):
+        for c in range(hp.num_r):
+            n = hp.get("volume", c).size
+            v = np.full(n, 0.37 + trial) if uniform else rng.uniform(0.5, 2.0, n)
+            hp.set("volume", c, v)
+            orf.set("volume", c, v)
+        pop = k.sweep_solver_population(sdoms)
+        assert abs(pop - k.population()) <= 1e-13 * abs(pop)
+        orf.sweep_solver()
+        _, comp = orf.population()
+        assert abs(pop - comp) <= TOL_REF * abs(comp), (trial, pop, comp)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_iterations_c0(ctx, layout, mode):
