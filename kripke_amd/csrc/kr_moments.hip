@@ -1017,8 +1017,9 @@ static void lplus_launch(const LaunchCtx &L, dim3 grid, int db) {
 // mode: 0 overwrite, 1 accumulate, 2 adjoint (overwrite)
 static int ltimes_shape(const LaunchCtx &L, int mode, const char *what, bool lplus) {
   dim3 grid(grid_for(L.max_gz, 256, 1 << 30), L.num_pr);
-  // LPlusTimes writes 2 directions per batch: with 8 in flight per thread the
-  // C1 kernel took 10.5 ms, with 4 about 7.9 ms and with 1 or 2 about 7.4 ms
+  // LPlusTimes writes 2 directions per batch: on 64^3 zones x 128 groups x
+  // 128 directions at P4 the kernel took 10.5 ms with 8 in flight per thread,
+  // about 7.9 ms with 4 and about 7.4 ms with 1 or 2
   const int db = batch_of(L.nd, lplus ? 2 : 16);
   switch (L.num_moments) {
 #define X(m)                                                     \

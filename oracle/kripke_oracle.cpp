@@ -327,7 +327,7 @@ static bool create_quadrature(const ko_input &in, std::vector<QuadPoint> &dirs) 
 }
 
 /* ------------------------------------------------------------------------ */
-/* Seeded generator of the configs: SplitMix64, boxes and cross sections.  */
+/* Seeded generator of the synthetic inputs: SplitMix64, boxes and cross sections. */
 /* Same draw order in kripke_amd/csrc/host/generate.cpp.                   */
 /* ------------------------------------------------------------------------ */
 struct SplitMix64 {

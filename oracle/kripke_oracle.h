@@ -30,7 +30,7 @@ extern "C" {
 
 /* Same field order and meaning as kr_input_vars in include/kripke_amd.h.
  * Mirrors InputVariables (src/Kripke/InputVariables.h:19-46) plus the
- * seeded-geometry knobs of the configs. */
+ * seeded-geometry knobs of the synthetic inputs. */
 typedef struct ko_input {
   int32_t nx, ny, nz;
   int32_t num_directions;
