@@ -368,6 +368,43 @@ __global__ __launch_bounds__(256) void k_scatter_diag(const DevPR *__restrict__ 
   }
 }
 
+// k_scatter_diag, overwrite form, for the direction-fastest nestings (GZD,
+// ZGD), where a column's M moments are adjacent in phi_out ([column][nm]):
+// a block fills DF_CB columns' moments into an LDS tile and writes the
+// [DF_CB][M] slab out with coalesced stores (a thread-per-column store of M
+// adjacent values touches a different line per lane).  Same values as
+// k_scatter_diag.
+template <bool ADJ>
+__global__ __launch_bounds__(DF_CB) void k_scatter_diag_dfast(const DevPR *__restrict__ prs,
+                                                              const DevR *__restrict__ rs,
+                                                              const double *__restrict__ sigs, SigsStrides ss, int M,
+                                                              const int32_t *__restrict__ flags) {
+  extern __shared__ double tile[];
+  if (sigs_coupled(flags)) return;
+  const DevPR &P = prs[blockIdx.y];
+  const DevR &R = rs[P.r];
+  const int64_t C = (int64_t)P.ng * P.nz;
+  const int64_t c0 = (int64_t)blockIdx.x * DF_CB;
+  if (c0 >= C) return;
+  const int cn = (int)min<int64_t>(DF_CB, C - c0);
+  const int ts = M | 1;
+  const bool zfast = P.mz <= P.mg;
+  double *dst = ADJ ? P.phi : P.phi_out;
+  const double *src = ADJ ? P.phi_out : P.phi;
+  const int tid = threadIdx.x;
+  if (tid < cn) {
+    int32_t g, z;
+    gz_of(c0 + tid, P.ng, P.nz, zfast, g, z);
+    const int64_t po = g * P.mg + z * P.mz;
+    const int64_t gg = P.glo + g;
+    const double t = sigs_zone(sigs, ss, R, z, 0, gg, gg) * src[po];
+    tile[tid * ts] = 0.0 + t;
+    for (int nm = 1; nm < M; ++nm) tile[tid * ts + nm] = 0.0 + 0.0;
+  }
+  __syncthreads();
+  tile_to_slab(gp(dst) + c0 * M, tile, ts, cn * M, M);
+}
+
 // General sigs (group coupling, anisotropic moments).  The zone's matrix
 // sigs_z(n,g,gp) = sum over its mixed elements of frac * sigs(mat,n,g,gp) is
 // linear in the materials, so
@@ -1064,12 +1101,27 @@ int launch_scattering(const LaunchCtx &L, const int32_t *pr_gset, const int32_t 
                      L.sigs_dims[1], L.sigs_dims[2], L.sigs_flags);
   KR_HIP_CHECK(hipGetLastError());
   dim3 grid(grid_for(L.max_gz), L.num_pr);
-  if (adjoint)
+  const size_t tile_lds = (size_t)DF_CB * (L.num_moments | 1) * sizeof(double);
+  if (L.dfast && !accumulate && tile_lds <= L.lds_max) {
+    // phi_out is [column][nm] in these nestings: coalesced through an LDS tile
+    const dim3 g2((unsigned)((L.max_gz + DF_CB - 1) / DF_CB), L.num_pr);
+    int rc;
+    if (tile_lds > 65536 && ((rc = allow_lds((const void *)k_scatter_diag_dfast<true>, tile_lds)) ||
+                             (rc = allow_lds((const void *)k_scatter_diag_dfast<false>, tile_lds))))
+      return rc;
+    if (adjoint)
+      hipLaunchKernelGGL(k_scatter_diag_dfast<true>, g2, dim3(DF_CB), tile_lds, L.stream, L.pr, L.r, L.sigs, ss,
+                         L.num_moments, (const int32_t *)L.sigs_flags);
+    else
+      hipLaunchKernelGGL(k_scatter_diag_dfast<false>, g2, dim3(DF_CB), tile_lds, L.stream, L.pr, L.r, L.sigs, ss,
+                         L.num_moments, (const int32_t *)L.sigs_flags);
+  } else if (adjoint) {
     hipLaunchKernelGGL(k_scatter_diag<true>, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments,
                        accumulate, (const int32_t *)L.sigs_flags);
-  else
+  } else {
     hipLaunchKernelGGL(k_scatter_diag<false>, grid, dim3(256), 0, L.stream, L.pr, L.r, L.sigs, ss, L.num_moments,
                        accumulate, (const int32_t *)L.sigs_flags);
+  }
   KR_HIP_CHECK(hipGetLastError());
   // Dense: coefficient re-layout, then for each source chunk (work-list
   // order) and each destination chunk on the same zone set
