@@ -107,6 +107,46 @@ def test_batch_mixed_extents_and_coupled_sigs(ctx):
         hp.close()
 
 
+# (zones, zone sets, GL quadrature) that the tuned shapes do not divide:
+# one-zone subdomains, odd rows, single-zone axes, one and three directions
+# per direction set, and a 2x2x2 pipeline of 5x3x3 subdomains.
+ODD_EXTENTS = [((1, 1, 1), (1, 1, 1), (4, 2)), ((3, 1, 2), (1, 1, 2), (4, 6)), ((10, 6, 6), (2, 2, 2), (4, 2)),
+               ((7, 5, 3), (1, 1, 1), (4, 6)), ((1, 4, 3), (1, 2, 1), (4, 6))]
+
+
+def test_batch_odd_extents(ctx):
+    """Problems of those extents in all six nestings in one batch, from a
+    seeded psi (so that scattering feeds the sweep where the source box
+    misses the zones): populations, phi and psi equal each problem's own
+    runs bit for bit."""
+    layouts = ["DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD"]
+    cfgs = [dict(zones=z, zset=zs, quad=q, groups=3, legendre=2, layout=layouts[i % 6], dset=8, gset=1, geometry=1,
+                 seed=500 + i) for i, (z, zs, q) in enumerate(ODD_EXTENTS * 2)]
+
+    def seeded(hps):
+        rng = np.random.default_rng(9)
+        for hp in hps:
+            for s in range(hp.num_sdoms):
+                hp.set("psi", s, 1.0 + rng.standard_normal(hp.get("psi", s).size))
+        return hps
+
+    ref = seeded([ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs])
+    want = np.array([[hp.iterate(1) for _ in range(3)] for hp in ref])
+    hps = seeded([ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs])
+    b = ka.Batch(ctx, hps)
+    got = b.iterate(3)
+    for i in range(len(cfgs)):
+        assert np.all(want[i] != 0.0)
+        assert np.array_equal(got[i], want[i]), "problem %d (%s): %s vs %s" % (i, cfgs[i], got[i], want[i])
+        for k in range(hps[i].num_pr):
+            assert np.array_equal(hps[i].get("phi", k), ref[i].get("phi", k)), "problem %d chunk %d" % (i, k)
+        for k in range(hps[i].num_sdoms):
+            assert np.array_equal(hps[i].get("psi", k), ref[i].get("psi", k)), "problem %d sdom %d" % (i, k)
+    b.close()
+    for hp in hps + ref:
+        hp.close()
+
+
 def test_batch_rejects_bad_input(ctx):
     hp = ka.HostProblem(ka.input_vars(**mixed_problems(1)[0]), ctx)
     with pytest.raises(ka.KripkeError):
