@@ -149,12 +149,16 @@ PLANES = ("i_plane", "j_plane", "k_plane")
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
-@pytest.mark.parametrize("zones,zset", [((12, 6, 10), (2, 2, 2)), ((8, 8, 8), (1, 1, 1)), ((128, 80, 128), (2, 2, 2))])
+@pytest.mark.parametrize("zones,zset", [((12, 6, 10), (2, 2, 2)), ((8, 8, 8), (1, 1, 1)), ((128, 80, 128), (2, 2, 2)),
+                                        ((1, 1, 1), (1, 1, 1)), ((7, 5, 3), (1, 1, 1)), ((1, 4, 3), (1, 2, 1)),
+                                        ((10, 6, 6), (2, 2, 2))])
 def test_sweep_transpose_seeded(ctx, layout, zones, zset):
     """The sweep as a map rhs -> (psi, faces leaving the domain) and its
     transpose, the seeded adjoint sweep (the faces' cotangent enters as
     adjoint inflow on the domain boundary):
-    <y, H x> + <fb, B x> = <H^T y + B^T fb, x>."""
+    <y, H x> + <fb, B x> = <H^T y + B^T fb, x>.  Besides the tuned
+    extents: a one-zone subdomain, odd rows, a single-zone axis and a 2x2x2
+    pipeline of 5x3x3 subdomains."""
     big = zones[0] > 64
     hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=2 if big else 4, zones=zones,
                                        quad=(4, 2) if big else (4, 4), legendre=1))
