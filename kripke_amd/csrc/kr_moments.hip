@@ -198,6 +198,24 @@ __device__ __forceinline__ void slab_to_tile(double *tile, int ts, const gmem<do
   }
 }
 
+// One slab held in registers: element tid + b*DF_CB in v[b] (NB*DF_CB >= total).
+constexpr int DF_PF = 16;  // slab elements per thread of the register prefetch
+__device__ __forceinline__ void slab_to_regs(double (&v)[DF_PF], const gmem<double> *src, int total) {
+#pragma unroll
+  for (int b = 0; b < DF_PF; ++b) {
+    const int e = threadIdx.x + b * DF_CB;
+    v[b] = e < total ? __builtin_nontemporal_load(src + e) : 0.0;
+  }
+}
+__device__ __forceinline__ void regs_to_tile(double *tile, int ts, const double (&v)[DF_PF], int total, int w) {
+  SlabWalk sw(threadIdx.x, w);
+#pragma unroll
+  for (int b = 0; b < DF_PF; ++b) {
+    if (threadIdx.x + b * DF_CB < total) tile[sw.col * ts + sw.k] = v[b];
+    sw.next(w);
+  }
+}
+
 __device__ __forceinline__ void tile_to_slab(gmem<double> *dst, const double *tile, int ts, int total, int w) {
   SlabWalk sw(threadIdx.x, w);
   for (int e = threadIdx.x; e < total; e += DF_CB) {
@@ -229,11 +247,12 @@ __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restric
 #pragma unroll
     for (int nm = 0; nm < M; ++nm) acc[nm] = 0.0;
   }
-  for (int k = 0; k < P.sdom_count; ++k) {
+  auto slab = [&](int k) {
     const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
-    slab_to_tile<true>(tile, ts, gp(ADJ ? S.rhs : S.psi) + c0 * nd, cn * nd, nd);
-    __syncthreads();
-    const cmem<double> *ell = cp(ADJ ? S.ell_plus : S.ell);
+    return gp(ADJ ? S.rhs : S.psi) + c0 * nd;
+  };
+  auto fold = [&](int k) {
+    const cmem<double> *ell = cp(ADJ ? sd[pr_sdoms[P.sdom_begin + k]].ell_plus : sd[pr_sdoms[P.sdom_begin + k]].ell);
     if (tid < cn) {
       for (int d = 0; d < nd; ++d) {
         const double p = tile[tid * ts + d];
@@ -241,7 +260,26 @@ __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restric
         for (int nm = 0; nm < M; ++nm) acc[nm] = __fma_rn(ell[d * M + nm], p, acc[nm]);
       }
     }
-    __syncthreads();
+  };
+  if (nd <= DF_PF) {
+    // the next subdomain's slab is loaded into registers while this one's
+    // FMAs run from the tile, so the block's loads stay in flight
+    double v[DF_PF];
+    slab_to_regs(v, slab(0), cn * nd);
+    for (int k = 0; k < P.sdom_count; ++k) {
+      regs_to_tile(tile, ts, v, cn * nd, nd);
+      __syncthreads();
+      if (k + 1 < P.sdom_count) slab_to_regs(v, slab(k + 1), cn * nd);
+      fold(k);
+      __syncthreads();
+    }
+  } else {
+    for (int k = 0; k < P.sdom_count; ++k) {
+      slab_to_tile<true>(tile, ts, slab(k), cn * nd, nd);
+      __syncthreads();
+      fold(k);
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int nm = 0; nm < M; ++nm) tile[tid * ts + nm] = acc[nm];

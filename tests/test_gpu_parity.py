@@ -60,6 +60,27 @@ def test_ltimes(ctx, layout):
     _assert_chunks(hp, of, None, "phi", hp.num_pr)
 
 
+@pytest.mark.parametrize("layout", ["DGZ", "GZD", "ZGD"])
+def test_ltimes_lplustimes_many_directions(ctx, layout):
+    """24 directions per direction set (GL 8 x 24): the direction-fastest
+    kernels stage slabs wider than their 16-per-thread register prefetch."""
+    kw = kwargs_for(layout, zset=(2, 1, 1), zones=(6, 4, 4), groups=4, quad=(8, 24))
+    hp, of, orf = _trio(ctx, **kw)
+    rng = np.random.default_rng(18)
+    seeded_fill(hp, [of, orf], "psi", hp.num_sdoms, rng)
+    seeded_fill(hp, [of, orf], "phi", hp.num_pr, rng)
+    k = hp.kernels()
+    k.ltimes(accumulate=True)
+    of.ltimes()
+    orf.ltimes()
+    _assert_chunks(hp, of, orf, "phi", hp.num_pr)
+    seeded_fill(hp, [of, orf], "phi_out", hp.num_pr, rng)
+    k.lplustimes(accumulate=False)
+    of.kconst("rhs", 0.0)
+    of.lplustimes()
+    _assert_chunks(hp, of, None, "rhs", hp.num_sdoms)
+
+
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_lplustimes(ctx, layout):
     kw = kwargs_for(layout, gset=2, zset=(1, 2, 1), groups=8)
