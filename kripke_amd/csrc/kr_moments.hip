@@ -252,7 +252,8 @@ __global__ __launch_bounds__(DF_CB) void k_ltimes_dfast(const DevSdom *__restric
     return gp(ADJ ? S.rhs : S.psi) + c0 * nd;
   };
   auto fold = [&](int k) {
-    const cmem<double> *ell = cp(ADJ ? sd[pr_sdoms[P.sdom_begin + k]].ell_plus : sd[pr_sdoms[P.sdom_begin + k]].ell);
+    const DevSdom &S = sd[pr_sdoms[P.sdom_begin + k]];
+    const cmem<double> *ell = cp(ADJ ? S.ell_plus : S.ell);
     if (tid < cn) {
       for (int d = 0; d < nd; ++d) {
         const double p = tile[tid * ts + d];
