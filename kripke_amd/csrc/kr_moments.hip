@@ -162,7 +162,11 @@ __global__ __launch_bounds__(256) void k_lplustimes(const DevSdom *__restrict__ 
 // tile with coalesced loads and stores; each thread then runs its column from
 // LDS with the same FMA order as the column kernels above.
 // ---------------------------------------------------------------------------
-constexpr int DF_CB = 128;  // columns per block
+// One wave per block: its barriers cost next to nothing.  On 64^3 zones x
+// 128 groups x 128 directions (GZD) 64 columns took LPlusTimes from 8.7 to
+// 8.0-8.2 ms and LTimes from 7.6-7.8 to 7.4 ms against 128; 256 took
+// LPlusTimes to 10.9 ms.
+constexpr int DF_CB = 64;  // columns per block
 
 // Slab [n][w] (row-major, n <= DF_CB rows) <-> tile rows of stride ts.
 // Element e = tid + i*DF_CB maps to (e / w, e % w), stepped without division.
