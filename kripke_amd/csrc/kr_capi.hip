@@ -259,8 +259,9 @@ int check_unit_spans(const kr_problem *p, const std::vector<SweepUnit> &units, c
 // eight: its per-unit times vary, and a level of many short units balances
 // across the CUs and XCDs.  On 64^3 zones x 128 groups x 128 directions in
 // 2x2x2 zone sets (R = 12 -> 1 on the middle levels) its sweep took
-// 32.8 -> 22.9 ms (DZG) and 30.7 -> 21.8 ms (GZD), while R other than one
-// unit per slot made the row-scan sweep slower.
+// 32.8 -> 22.9 ms (DZG) and 30.7 -> 21.8 ms (GZD) in the autotuner's
+// sweep-only timings, while R other than one unit per slot made the
+// row-scan sweep slower.
 int32_t pick_R(int64_t total_pgs, const SweepShape &sh) {
   const int nt = sh.kind == 1 ? sh.RG * sh.PK : sh.Q * sh.PJ * sh.PK;
   const int64_t slots = 256LL * std::max(1, 1024 / std::max(nt, 1));  // resident workgroups, one per 1024 threads

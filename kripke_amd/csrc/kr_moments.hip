@@ -202,7 +202,7 @@ __device__ __forceinline__ void slab_to_tile(double *tile, int ts, const gmem<do
   }
 }
 
-// One slab held in registers: element tid + b*DF_CB in v[b] (NB*DF_CB >= total).
+// One slab held in registers: element tid + b*DF_CB in v[b] (DF_PF*DF_CB >= total).
 constexpr int DF_PF = 16;  // slab elements per thread of the register prefetch
 __device__ __forceinline__ void slab_to_regs(double (&v)[DF_PF], const gmem<double> *src, int total) {
 #pragma unroll
