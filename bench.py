@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 METRIC = "sweep unknowns/s FP64 (64^3z x128g x128d), % HBM peak; adjoint/forward cost"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 SWEEP_STAGES = 4   # zone-set dependency levels of a 2x2x2 decomposition: one sweep dispatch each
+NESTINGS = ["DGZ", "DZG", "GDZ", "GZD", "ZDG", "ZGD"]
+NESTING_STEPS, NESTING_WARMUP = 5, 2  # the other nestings of the workload, each timed on its own
 
 WORKLOAD = dict(zones=(64, 64, 64), groups=128, quad=(8, 16), legendre=4, dset=8, gset=1, zset=(2, 2, 2),
                 geometry=1, seed=20261202)
@@ -60,7 +62,10 @@ def measured_traffic(layout):
     return None
 
 
-def run_gpu(args, layout):
+def run_gpu(layout, steps, warmup):
+    """Times `steps` source iterations of the workload in `layout` after
+    `warmup` untimed ones; then a breakdown pass with the hipEvent timers,
+    outside the timed region."""
     import kripke_amd as ka
     ctx = ka.Context(0)
     t0 = time.time()
@@ -68,15 +73,14 @@ def run_gpu(args, layout):
     setup_s = time.time() - t0
     k = hp.kernels()
     hp.init_psi()
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         hp.iterate(1)
     ctx.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         hp.iterate(1)  # ends with the population read-back (a host sync per iteration)
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    # breakdown pass with hipEvent timers, outside the timed region
     k.set_timing(True)
     k.timer_reset()
     nb = 2
@@ -88,6 +92,29 @@ def run_gpu(args, layout):
     hp.close()
     ctx.close()
     return elapsed, setup_s, timers, nb
+
+
+def nesting_figures(layout, steps, elapsed, timers, nb):
+    """Whole-iteration FOM, the metric path (sweep + LTimes + LPlusTimes)
+    against its algorithmic bytes, and the sweep kernel's own fraction."""
+    nz = 64 ** 3
+    ab = algorithmic_bytes(128, 128, nz, 25)
+    sweep_t, sweep_n = timers["k_sweep"]
+    lt_t, _ = timers["LTimes"]
+    lp_t, _ = timers["LPlusTimes"]
+    ss_t, _ = timers["SweepSolver"]
+    path_t = (lt_t + lp_t + ss_t) / nb
+    path_bytes = ab["ltimes"] + ab["lplustimes"] + ab["sweep"]
+    return {
+        "unknowns_per_s": ab["N"] * steps / elapsed, "ms_per_iteration": elapsed / steps * 1e3,
+        "metric_path": {"ms_per_iteration": path_t * 1e3, "alg_bytes_per_iteration": path_bytes,
+                        "hbm_frac": path_bytes / path_t / HBM_PEAK if path_t > 0 else None,
+                        "unknowns_per_s": ab["N"] / path_t if path_t > 0 else None},
+        "sweep_kernel": {"kernel": "k_sweep_rows" if layout in ("DGZ", "GDZ") else "k_sweep",
+                         "avg_launch_ms": sweep_t / max(sweep_n, 1) * 1e3,
+                         "hbm_frac": (ab["sweep"] * nb) / sweep_t / HBM_PEAK if sweep_t > 0 else None},
+        "breakdown_ms": {n: timers[n][0] / nb * 1e3 for n in timers},
+    }
 
 
 ADJ_WORKLOAD = dict(zones=(32, 32, 32), groups=32, quad=(8, 12), legendre=2, dset=8, gset=1, zset=(1, 1, 1),
@@ -288,6 +315,13 @@ def run_batched():
     tune_s = time.perf_counter() - t0
     unknowns = sum(c["zones"][0] * c["zones"][1] * c["zones"][2] * c["groups"] * c["quad"][0] * c["quad"][1]
                    for c in cfgs)
+    # compulsory HBM bytes of one iteration of every problem (the metric path's 32N + 16P + 8GZ, each problem's
+    # own N, P and G*Z): the graph's and the per-problem leg's fractions of the HBM peak
+    alg_bytes = 0
+    for c in cfgs:
+        nzc = c["zones"][0] * c["zones"][1] * c["zones"][2]
+        b = algorithmic_bytes(c["quad"][0] * c["quad"][1], c["groups"], nzc, (c["legendre"] + 1) ** 2)
+        alg_bytes += b["ltimes"] + b["lplustimes"] + b["sweep"]
     # problem by problem
     for hp in hps:
         hp.init_psi()
@@ -317,6 +351,9 @@ def run_batched():
                                                                                       BATCH_ITERATIONS),
             "unknowns_per_iteration": unknowns, "graph_ms": graph_s * 1e3, "per_problem_ms": seq_s * 1e3,
             "speedup_vs_per_problem": seq_s / graph_s, "graph_unknowns_per_s": unknowns * BATCH_ITERATIONS / graph_s,
+            "alg_bytes_per_iteration": alg_bytes,
+            "graph_hbm_frac": alg_bytes * BATCH_ITERATIONS / graph_s / HBM_PEAK,
+            "per_problem_hbm_frac": alg_bytes * BATCH_ITERATIONS / seq_s / HBM_PEAK,
             "populations_equal_per_problem_runs": same, "nestings": {k: v for k, v in nest.items() if v},
             "autotune_nesting_s": nest_s, "autotune_s": tune_s}
 
@@ -347,6 +384,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--layout", default="DGZ")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-nestings", action="store_true", help="skip the other five nestings of the workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch leg")
     ap.add_argument("--batch-only", action="store_true",
@@ -363,30 +401,21 @@ def main():
         print(json.dumps({"batched": run_batched()}))
         return
 
-    elapsed, setup_s, timers, nb = run_gpu(args, args.layout)
-    nz = 64 ** 3
-    nd, ng, M = 128, 128, 25
-    ab = algorithmic_bytes(nd, ng, nz, M)
+    elapsed, setup_s, timers, nb = run_gpu(args.layout, args.steps, args.warmup)
+    fig = nesting_figures(args.layout, args.steps, elapsed, timers, nb)
+    ab = algorithmic_bytes(128, 128, 64 ** 3, 25)
     N = ab["N"]
-    ms = elapsed / args.steps * 1e3
-    value = N * args.steps / elapsed
-
     sweep_t, sweep_n = timers["k_sweep"]
-    lt_t, _ = timers["LTimes"]
-    lp_t, _ = timers["LPlusTimes"]
-    ss_t, _ = timers["SweepSolver"]
     # sweep kernel roofline: algorithmic bytes of all stage launches / their time
     sweep_achieved = (ab["sweep"] * nb) / sweep_t if sweep_t > 0 else None
-    path_t = (lt_t + lp_t + ss_t) / nb
-    path_bytes = ab["ltimes"] + ab["lplustimes"] + ab["sweep"]
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": fig["unknowns_per_s"],
         "unit": "unknowns/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": fig["ms_per_iteration"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -399,15 +428,22 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": sweep_achieved / HBM_PEAK if sweep_achieved else None,
                      "traffic": measured_traffic(args.layout),
-                     "kernel": "k_sweep_rows" if args.layout in ("DGZ", "GDZ") else "k_sweep",
+                     "kernel": fig["sweep_kernel"]["kernel"],
                      "alg_bytes_per_launch": ab["sweep"] / SWEEP_STAGES,
                      "avg_launch_ms": sweep_t / max(sweep_n, 1) * 1e3},
-        "metric_path": {"ms_per_iteration": path_t * 1e3, "alg_bytes_per_iteration": path_bytes,
-                        "hbm_frac": path_bytes / path_t / HBM_PEAK if path_t > 0 else None,
-                        "unknowns_per_s": N / path_t if path_t > 0 else None},
-        "breakdown_ms": {n: timers[n][0] / nb * 1e3 for n in timers},
+        "metric_path": fig["metric_path"],
+        "breakdown_ms": fig["breakdown_ms"],
         "setup_s": setup_s,
     }
+    if not args.no_nestings:
+        # the same workload in every nesting (the headline's own figures included once more, from its run above)
+        per = {args.layout: fig}
+        for lay in NESTINGS:
+            if lay != args.layout:
+                e, _, tm, b = run_gpu(lay, NESTING_STEPS, NESTING_WARMUP)
+                per[lay] = nesting_figures(lay, NESTING_STEPS, e, tm, b)
+        out["nestings"] = {"steps": NESTING_STEPS, "warmup": NESTING_WARMUP,
+                           "figures": {lay: per[lay] for lay in NESTINGS if lay in per}}
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint(args.layout)
         out["adjoint_batched"] = run_adjoint_batched(args.layout)
