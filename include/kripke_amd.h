@@ -139,7 +139,14 @@ int kr_sweep_subdomain(kr_problem *prob, int32_t sdom_id);
 /* SweepSolver (src/Kripke/SweepSolver.cpp:23-87) with the on-rank
  * SweepComm hand-off: sweeps every listed subdomain in dependency order,
  * vacuum faces read as zero, outgoing faces written straight into the
- * downwind subdomain's plane chunk.  Upwind subdomains must be listed. */
+ * downwind subdomain's plane chunk.  Upwind subdomains must be listed.
+ * psi is the reference's.  The plane fields end differently: the reference
+ * leaves subdomain s's outgoing faces along an axis in s's own chunk and
+ * copies them downwind (ParallelComm.cpp:156-162); here they end in the
+ * downwind subdomain's chunk when s has a downwind neighbour on that axis
+ * (which so holds its incoming faces), and in s's own chunk on the domain
+ * boundary, as in the reference.  The reference's exact plane state comes
+ * from kr_sweep_subdomain plus kr_kcopy hand-offs in the queue order. */
 int kr_sweep_solver(kr_problem *prob, const int32_t *sdom_list, int32_t num);
 /* SweepSolver with block_jacobi = true (--pmethod bj): SweepSolver.cpp:23-87
  * with BlockJacobiComm (src/Kripke/ParallelComm/BlockJacobiComm.cpp:23-116)
@@ -236,7 +243,9 @@ int kr_problem_autotune(kr_problem *prob, int32_t *choice);
  * graph is captured on the first call and re-captured when niter changes, a
  * problem is re-tuned, or a problem's sigs changes between group-diagonal
  * and coupled; the problems' fields must not be re-registered while the
- * batch lives.  Blocks until the values are on the host. */
+ * batch lives.  Member problems must outlive the batch: once one is
+ * destroyed, kr_batch_iterate fails with KR_ERR_INVALID (kr_batch_destroy
+ * still releases the batch).  Blocks until the values are on the host. */
 typedef struct kr_batch kr_batch;
 int kr_batch_create(kr_context *ctx, kr_problem *const *problems, const double *source_strength, int32_t n,
                     kr_batch **out);

@@ -379,13 +379,20 @@ int main(int argc, char **argv) {
     const double iter_time = solve_time / (double)num_iter;
     const double grind_time = iter_time / (double)num_unknowns;
     const double throughput = (double)num_unknowns / iter_time;
+    // the on-device pipeline (--mode pipeline) sweeps whole zone-set stages
+    // and never calls Kernel::sweepSubdomain, so there is no SweepSubdomain
+    // time to put over SweepSolver's
+    const bool have_sdom_timer = timing->getCount("SweepSubdomain") > 0;
     const double sweep_eff = 100.0 * timing->getTotal("SweepSubdomain") / timing->getTotal("SweepSolver");
     printf("\n");
     printf("Figures of Merit\n");
     printf("================\n\n");
     printf("  Throughput:         %e [unknowns/(second/iteration)]\n", throughput);
     printf("  Grind time :        %e [(seconds/iteration)/unknowns]\n", grind_time);
-    printf("  Sweep efficiency :  %4.5lf [100.0 * SweepSubdomain time / SweepSolver time]\n", sweep_eff);
+    if (have_sdom_timer)
+      printf("  Sweep efficiency :  %4.5lf [100.0 * SweepSubdomain time / SweepSolver time]\n", sweep_eff);
+    else
+      printf("  Sweep efficiency :  n/a [no SweepSubdomain calls: the pipeline sweeps whole zone-set stages]\n");
     printf("  Number of unknowns: %lu\n", (unsigned long)num_unknowns);
   } catch (const std::exception &e) {
     printf("ERROR: %s\n", e.what());

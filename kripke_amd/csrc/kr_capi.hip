@@ -86,7 +86,14 @@ struct kr_problem {
   int timing = 0;
   std::vector<TimedCall> pending;
   std::map<std::string, std::pair<double, int64_t>> timers;
+  // batches this problem belongs to (kr_batch_create); destroying the
+  // problem marks them unusable
+  std::vector<kr_batch *> batches;
 };
+
+// A batch whose member problem is destroyed keeps its handle valid (so that
+// kr_batch_destroy still works) but refuses kr_batch_iterate.
+static void batch_forget_problem(kr_batch *b, kr_problem *p);
 
 namespace {
 
@@ -681,6 +688,8 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
 int kr_problem_destroy(kr_problem *p) {
   if (!p) return KR_OK;
   if (p->ctx) hipStreamSynchronize(p->ctx->stream);
+  for (kr_batch *b : p->batches) batch_forget_problem(b, p);
+  p->batches.clear();
   for (auto &tc : p->pending) {
     hipEventDestroy(tc.start);
     hipEventDestroy(tc.stop);
@@ -755,8 +764,11 @@ static int sweep_solver_impl(kr_problem *p, const int32_t *list, int32_t num, do
       const int64_t n = p->pipe_level_begin.empty() ? 0
                                                     : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
       if (n > p->pop_units_cap) {
+        // pointer and capacity are cleared first: a failed allocation leaves
+        // no freed buffer behind a non-zero capacity
         if (p->d_pop_units) KR_HIP_CHECK(hipFree(p->d_pop_units));
         p->d_pop_units = nullptr;
+        p->pop_units_cap = 0;
         KR_HIP_CHECK(hipMalloc((void **)&p->d_pop_units, std::max<int64_t>(n, 1) * sizeof(double)));
         p->pop_units_cap = n;
       }
@@ -793,6 +805,8 @@ static int sweep_solver_impl(kr_problem *p, const int32_t *list, int32_t num, do
   KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
   if ((int32_t)u.size() > p->units_tmp_cap) {
     if (p->d_units_tmp) KR_HIP_CHECK(hipFree(p->d_units_tmp));
+    p->d_units_tmp = nullptr;
+    p->units_tmp_cap = 0;
     KR_HIP_CHECK(hipMalloc((void **)&p->d_units_tmp, u.size() * sizeof(SweepUnit)));
     p->units_tmp_cap = (int32_t)u.size();
   }
@@ -848,6 +862,7 @@ int kr_sweep_solver_bj(kr_problem *p, const int32_t *list, int32_t num) {
   if ((int32_t)u.size() > p->units_tmp_cap) {
     if (p->d_units_tmp) KR_HIP_CHECK(hipFree(p->d_units_tmp));
     p->d_units_tmp = nullptr;
+    p->units_tmp_cap = 0;
     KR_HIP_CHECK(hipMalloc((void **)&p->d_units_tmp, u.size() * sizeof(SweepUnit)));
     p->units_tmp_cap = (int32_t)u.size();
   }
@@ -1212,9 +1227,16 @@ struct kr_batch {
   hipStream_t branch[BATCH_BRANCHES] = {};
   hipEvent_t join[BATCH_BRANCHES] = {};
   hipEvent_t fork = nullptr;
+  bool member_destroyed = false;     // a member problem was destroyed while the batch lived
 };
 
 }  // extern "C"
+
+static void batch_forget_problem(kr_batch *b, kr_problem *p) {
+  for (kr_problem *&q : b->probs)
+    if (q == p) q = nullptr;
+  b->member_destroyed = true;
+}
 
 // Enqueues launch(stream, i) for i < n on BATCH_BRANCHES streams forked from
 // s0 and joined back (graph branches when captured): independent launches of
@@ -1259,6 +1281,8 @@ static void batch_free_tables(kr_batch *b) {
 int kr_batch_destroy(kr_batch *b) {
   if (!b) return KR_OK;
   if (b->ctx) hipStreamSynchronize(b->ctx->stream);
+  for (kr_problem *q : b->probs)
+    if (q) q->batches.erase(std::remove(q->batches.begin(), q->batches.end(), b), q->batches.end());
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   if (b->d_pop) hipFree(b->d_pop);
@@ -1306,6 +1330,7 @@ int kr_batch_create(kr_context *ctx, kr_problem *const *problems, const double *
     kr_batch_destroy(b);
     return fail(KR_ERR_HIP, "kr_batch_create: stream/event creation failed");
   }
+  for (kr_problem *q : b->probs) q->batches.push_back(b);
   *out = b;
   return KR_OK;
 }
@@ -1661,6 +1686,7 @@ static int batch_capture(kr_batch *b, int32_t niter, const std::vector<int32_t> 
   if (need > b->pop_cap) {
     if (b->d_pop) KR_HIP_CHECK(hipFree(b->d_pop));
     b->d_pop = nullptr;
+    b->pop_cap = 0;
     KR_HIP_CHECK(hipMalloc((void **)&b->d_pop, need * sizeof(double)));
     b->pop_cap = need;
   }
@@ -1702,6 +1728,8 @@ static int batch_capture(kr_batch *b, int32_t niter, const std::vector<int32_t> 
 
 int kr_batch_iterate(kr_batch *b, int32_t niter, double *populations) {
   if (!b || niter <= 0 || !populations) return fail(KR_ERR_INVALID, "kr_batch_iterate: bad argument");
+  if (b->member_destroyed)
+    return fail(KR_ERR_INVALID, "kr_batch_iterate: a member problem was destroyed while the batch lived");
   int rc;
   bool rebuild = b->versions.size() != b->probs.size();
   for (size_t i = 0; !rebuild && i < b->probs.size(); ++i) rebuild = b->versions[i] != b->probs[i]->sched_version;

@@ -993,6 +993,7 @@ __global__ __launch_bounds__(256) void k_pop_partial(const DevSdom *__restrict__
   const uint32_t nmid = ex[o1], nfast = ex[o2];
   double acc = 0.0;
   for (int64_t e = B.begin + threadIdx.x; e < B.end; e += blockDim.x) {
+    // exact: kr_problem_create rejects subdomain chunks above 2^31 elements
     const uint32_t ue = (uint32_t)e;
     const uint32_t f = ue % nfast, t = ue / nfast;
     const uint32_t m = t % nmid, s = t / nmid;

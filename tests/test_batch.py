@@ -206,3 +206,23 @@ def test_autotune_nesting(ctx):
         assert abs(pa - pb) <= 1e-12 * abs(pa)
     a.close()
     b.close()
+
+
+def test_batch_refuses_after_member_destroyed(ctx):
+    """A member problem destroyed while its batch lives: kr_batch_iterate
+    fails with KR_ERR_INVALID (no launch over freed tables), and the batch
+    is still released cleanly."""
+    cfgs = mixed_problems(2, seed0=700)
+    hps = [ka.HostProblem(ka.input_vars(**c), ctx) for c in cfgs]
+    for hp in hps:
+        hp.init_psi()
+    b = ka.Batch(ctx, hps)
+    b.iterate(1)
+    hps[1].close()
+    with pytest.raises(ka.KripkeError) as e:
+        b.iterate(1)
+    assert e.value.code == 1 and "destroyed" in str(e.value)
+    b.close()
+    # the surviving problem still runs on its own
+    assert np.isfinite(hps[0].iterate(1))
+    hps[0].close()

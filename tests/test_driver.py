@@ -86,6 +86,11 @@ def test_counts_match_oracle(layout, mode, pmethod):
     assert re.search(r"SweepSolver\s+3\s", out)
     fom = re.search(r"Throughput:\s+(\S+) \[unknowns/\(second/iteration\)\]", out)
     assert fom and float(fom.group(1)) > 0
+    eff = re.search(r"Sweep efficiency :\s+(\S+) \[", out)
+    if "SweepSubdomain" in names:
+        assert 0.0 < float(eff.group(1)) <= 100.0, eff.group(0)
+    else:  # the pipeline makes no per-subdomain sweep calls
+        assert eff.group(1) == "n/a", eff.group(0)
     assert "Number of unknowns: %d" % (8 * 16 * 8 ** 3) in out
     assert out.rstrip().endswith("END")
 

@@ -181,6 +181,20 @@ def test_sweep_solver_pipeline(ctx, layout):
     of.sweep_solver()
     orf.sweep_solver()
     _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
+    # plane state after the pipeline: the outgoing faces of subdomain s along
+    # an axis sit in its downwind neighbour's chunk when it has one (the faces
+    # go straight there), in its own chunk on the domain boundary; the
+    # reference leaves them in s's own chunk (ParallelComm.cpp:156-162)
+    for s in range(hp.num_sdoms):
+        info = of.sdom(s)
+        for a, name in enumerate(("i_plane", "j_plane", "k_plane")):
+            down = info["down%d" % a]
+            got = hp.get(name, down if down >= 0 else s)
+            want = of.get(name, s)
+            if layout in ZFAST:
+                assert rel_normwise(got, want) <= TOL_SCAN, (s, name)
+            else:
+                assert np.array_equal(got, want), (s, name)
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
@@ -448,7 +462,8 @@ def test_iterations_odd_extents(ctx, layout, case):
     the fma-order oracle.  With one zone along i the odd moments cancel by
     symmetry to rounding noise (~1e-17 against O(1) even moments), where a
     per-moment relative error divides noise by noise, so phi is held
-    normwise per chunk here."""
+    normwise per chunk there, and per moment wherever a subdomain has more
+    than one zone along i."""
     zones, zset, quad = ODD_EXTENT_CASES[case]
     kw = kwargs_for(layout, zones=zones, zset=zset, groups=3, quad=quad, legendre=2)
     hp, of, orf = _trio(ctx, **kw)
@@ -460,6 +475,10 @@ def test_iterations_odd_extents(ctx, layout, case):
         _, comp = orf.iterate()
         assert comp != 0.0
         assert abs(pg - comp) <= TOL_REF * abs(comp), "iteration %d population %.17g vs %.17g" % (it, pg, comp)
+        ni, nj, nk = (zones[a] // zset[a] for a in range(3))
+        if ni > 1:
+            norm, _ = phi_parity(hp, orf, hp.num_pr, layout, 9, 3, ni * nj * nk)
+            assert norm <= TOL_REF, "iteration %d phi per-moment normwise %.3e" % (it, norm)
         for c in range(hp.num_pr):
             a, b = hp.get("phi", c), of.get("phi", c)
             norm = rel_normwise(a, orf.get("phi", c))
