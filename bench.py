@@ -204,74 +204,80 @@ def run_adjoint_batched(layout):
     sigma_t(g,z) x (1 + 0.2 u_f) (u_f seeded uniform in [-1, 1]) and
     sigma_s = 0.5 sigma_t; the loss is the squared mismatch of the angular flux
     leaving the domain through all 6 faces against the exiting flux of a
-    seeded 'truth' field.  The fields run one after another through one
-    differentiable solver (all 8 at once would hold about 300 GB)."""
+    seeded 'truth' field.  All 8 fields go through one kr_diff_forward call
+    and one kr_diff_backward call (each field keeps its own parameters and
+    phi checkpoints; the fields take turns on one pair of problems, since all
+    8 live at once would hold about 300 GB)."""
     import numpy as np
     import kripke_amd as ka
     from kripke_amd.adjoint import DeviceBuffer, DifferentiableSolver, chunk_get, chunk_set
     ctx = ka.Context(0)
-    sv = DifferentiableSolver(ctx, ka.input_vars(layout=layout, **ADJB_WORKLOAD), ADJB_ITERATIONS, ADJB_CHECKPOINT)
+    F = ADJB_FIELDS
+    sv = DifferentiableSolver(ctx, ka.input_vars(layout=layout, **ADJB_WORKLOAD), ADJB_ITERATIONS, ADJB_CHECKPOINT,
+                              fields=F + 1)  # field F: the truth field
     sv.kf.autotune()
     sv.ka.autotune()
     rng = np.random.default_rng(20261203)
     sigt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("sigt"), sv.nparam)]
     bt = [DeviceBuffer(ctx, n) for n in sv.nparam]
     bs = [DeviceBuffer(ctx, n) for n in sv.nparam]
-    exits = sv.exit_faces()
 
-    def set_field(scale):
+    def set_field(f, scale):
         for b, c, v in zip(bt, bs, sigt):
-            pert = v * scale[:v.size] if scale is not None else v
+            pert = v * scale[:v.size]
             chunk_set(ctx, b.ptr.value, pert)
             chunk_set(ctx, c.ptr.value, 0.5 * pert)
-        sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+        sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs], field=f)
 
     nparam = max(sv.nparam)
-    set_field(1.0 + 0.2 * (2.0 * rng.random(nparam) - 1.0))  # the truth field
-    sv.forward()
-    target = {(sd, nm): chunk_get(ctx, p, n) for sd, nm, p, n in exits}
-    fb = {(sd, nm): DeviceBuffer(ctx, n) for sd, nm, _, n in exits}
-    scales = [1.0 + 0.2 * (2.0 * rng.random(nparam) - 1.0) for _ in range(ADJB_FIELDS)]
+    set_field(F, 1.0 + 0.2 * (2.0 * rng.random(nparam) - 1.0))  # the truth field
+    for f in range(F):
+        set_field(f, 1.0 + 0.2 * (2.0 * rng.random(nparam) - 1.0))
+    exits = sv.exit_faces()
+    ne = len(exits)
+    sv.forward(F)
+    target = [chunk_get(ctx, p, n) for _, _, p, n in exits]
+    faces = [DeviceBuffer(ctx, n) for _ in range(F) for _, _, _, n in exits]   # field-major exiting faces
+    fbar = [DeviceBuffer(ctx, n) for _ in range(F) for _, _, _, n in exits]    # their cotangents
+    gts = [DeviceBuffer(ctx, n) for _ in range(F) for n in sv.nparam]
+    gss = [DeviceBuffer(ctx, n) for _ in range(F) for n in sv.nparam]
     tf = tb = 0.0
     losses = []
-    t_all = time.perf_counter()
-    for f in range(ADJB_FIELDS + 1):  # field 0 once more first, untimed (warm-up)
-        set_field(scales[max(f - 1, 0)])
+    for rep in range(2):  # the first pass is a warm-up
         ctx.synchronize()
         t0 = time.perf_counter()
-        sv.forward()
+        sv.forward_fields(0, F, None, [b.ptr.value for b in faces])
         ctx.synchronize()
         t1 = time.perf_counter()
-        loss = 0.0
-        for sd, nm, p, n in exits:
-            d = chunk_get(ctx, p, n) - target[(sd, nm)]
-            loss += float(np.dot(d, d))
-            chunk_set(ctx, fb[(sd, nm)].ptr.value, 2.0 * d)
+        losses = []
+        for f in range(F):
+            loss = 0.0
+            for e in range(ne):
+                b = faces[f * ne + e]
+                dlt = chunk_get(ctx, b.ptr.value, b.n) - target[e]
+                loss += float(np.dot(dlt, dlt))
+                chunk_set(ctx, fbar[f * ne + e].ptr.value, 2.0 * dlt)
+            losses.append(loss)
         ctx.synchronize()
         t2 = time.perf_counter()
-        sv.backward(None, {key: b.ptr.value for key, b in fb.items()})
+        sv.backward_fields(0, F, None, [b.ptr.value for b in fbar], [b.ptr.value for b in gts],
+                           [b.ptr.value for b in gss])
         ctx.synchronize()
         t3 = time.perf_counter()
-        if f == 0:
-            t_all = time.perf_counter()
-            continue
-        tf += t1 - t0
-        tb += t3 - t2
-        losses.append(loss)
-    wall = time.perf_counter() - t_all
-    g = chunk_get(ctx, sv.param_ptrs("grad_sigt")[0], sv.nparam[0])
-    for b in bt + bs + list(fb.values()):
+        tf, tb, th = t1 - t0, t3 - t2, t2 - t1
+    g = chunk_get(ctx, gts[(F - 1) * sv.npr].ptr.value, sv.nparam[0])
+    for b in bt + bs + faces + fbar + gts + gss:
         b.free()
     sv.close()
     ctx.close()
     return {"workload": "%d fields x (48^3 zones, 64 groups, 128 directions GL 8x16, P3 (16 moments), %s), %d "
                         "iterations from psi=0, phi checkpoints every %d; parameters sigma_t and sigma_s per zone "
-                        "and group; loss on the exiting angular flux of all 6 boundary faces"
-                        % (ADJB_FIELDS, layout, ADJB_ITERATIONS, ADJB_CHECKPOINT),
-            "fields_per_s": ADJB_FIELDS / (tf + tb), "fields_per_s_wall": ADJB_FIELDS / wall,
-            "forward_ms_per_field": tf / ADJB_FIELDS * 1e3, "backward_ms_per_field": tb / ADJB_FIELDS * 1e3,
+                        "and group; loss on the exiting angular flux of all 6 boundary faces; all fields in one "
+                        "kr_diff_forward and one kr_diff_backward call" % (F, layout, ADJB_ITERATIONS, ADJB_CHECKPOINT),
+            "fields_per_s": F / (tf + tb), "fields_per_s_wall": F / (tf + tb + th),
+            "forward_ms_per_field": tf / F * 1e3, "backward_ms_per_field": tb / F * 1e3,
             "adjoint_forward_cost": tb / tf, "losses": losses, "grad_sigt_norm_last": float(np.linalg.norm(g)),
-            "note": "fields_per_s counts the forward and backward passes; fields_per_s_wall adds the host-side "
+            "note": "fields_per_s counts the forward and backward calls; fields_per_s_wall adds the host-side "
                     "loss and cotangent (exiting faces copied to the host)"}
 
 

@@ -141,12 +141,14 @@ int kr_sweep_subdomain(kr_problem *prob, int32_t sdom_id);
  * vacuum faces read as zero, outgoing faces written straight into the
  * downwind subdomain's plane chunk.  Upwind subdomains must be listed.
  * psi is the reference's.  The plane fields end differently: the reference
- * leaves subdomain s's outgoing faces along an axis in s's own chunk and
- * copies them downwind (ParallelComm.cpp:156-162); here they end in the
- * downwind subdomain's chunk when s has a downwind neighbour on that axis
- * (which so holds its incoming faces), and in s's own chunk on the domain
- * boundary, as in the reference.  The reference's exact plane state comes
- * from kr_sweep_subdomain plus kr_kcopy hand-offs in the queue order. */
+ * leaves every subdomain's outgoing faces in its own chunk (and copies them
+ * downwind first, ParallelComm.cpp:156-162).  Here subdomain t's chunk along
+ * an axis ends holding t's outgoing faces where t is on the domain's
+ * downwind boundary on that axis (as in the reference), its upwind
+ * neighbour's outgoing faces (t's incoming faces) where t has neighbours on
+ * both sides, and is not written where t has a downwind neighbour only.  The
+ * reference's exact plane state comes from kr_sweep_subdomain plus kr_kcopy
+ * hand-offs in the queue order. */
 int kr_sweep_solver(kr_problem *prob, const int32_t *sdom_list, int32_t num);
 /* SweepSolver with block_jacobi = true (--pmethod bj): SweepSolver.cpp:23-87
  * with BlockJacobiComm (src/Kripke/ParallelComm/BlockJacobiComm.cpp:23-116)
@@ -218,6 +220,52 @@ int kr_sweep_solver_adjoint_seeded(kr_problem *prob);
 int kr_sdom_exits(kr_problem *prob, int32_t sdom, int32_t *exits);
 int kr_grad_sigt(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
 int kr_grad_sigs_zonal(kr_problem *fwd, kr_problem *adj, double *const *grad, int32_t accumulate);
+
+/* ---- differentiable source iteration: forward/backward pair -------------- */
+/* No counterpart in the reference: K source iterations from psi = 0
+ * (src/Kripke/SteadyStateSolver.cpp:44-104) as a function of per-zone
+ * sigma_t(g,z) and isotropic group-diagonal scattering s(g,z) (one array per
+ * PR chunk in sigt_zonal's layout), with reverse-mode gradients:
+ *   phi_{n+1} = L H(sigma_t)^-1 L+ (S(s) phi_n + q),  phi_0 = 0,  output phi_K.
+ * fwd and adj are two problems registered on one context from the same
+ * inputs, each with its own psi, rhs, phi, phi_out and plane chunks (adj's
+ * hold the adjoint variables).  Both must outlive the kr_diff.  num_fields
+ * independent parameter sets share the two problems; each keeps its own
+ * copy of sigma_t and s and phi checkpoints every checkpoint_every
+ * iterations (memory per field: 2 parameter arrays + ceil(K/ck) phi
+ * snapshots; plus ck-1 snapshots for the recomputation), and backward
+ * recomputes the iterations between checkpoints.
+ *
+ * kr_diff_set_params   copies field f's parameters (num_pr device arrays
+ *                      each); returns when they are copied.
+ * kr_diff_forward      runs fields [first, first+num): phi_K of each into
+ *                      phi_k[(f-first)*num_pr + c] and the exiting faces of
+ *                      its last sweep into faces_out[(f-first)*num_exits + e]
+ *                      (either array, or any entry, may be NULL: not
+ *                      copied).  After the call the forward problem holds the
+ *                      last field's phi_K and exiting faces.
+ * kr_diff_backward     gradient of <phibar, phi_K> + <facebar, exiting faces>
+ *                      w.r.t. sigma_t and s for fields [first, first+num),
+ *                      overwriting grad_sigt/grad_s[(f-first)*num_pr + c];
+ *                      phibar/facebar NULL (or a NULL entry) = zero
+ *                      cotangent.  Needs the field's forward pass of its
+ *                      current parameters (KR_ERR_INVALID otherwise); leaves
+ *                      intermediate states in both problems' fields.
+ * Exit e (kr_diff_exit) is subdomain sdom's plane chunk along axis (0 = i,
+ * 1 = j, 2 = k) whose faces leave the domain, in (sdom, axis) order.
+ * Both passes write each field's sigma_t into the two problems'
+ * sigt_zonal chunks (the field the sweeps read).  Work is ordered on the
+ * context stream; kr_synchronize before reading outputs elsewhere. */
+typedef struct kr_diff kr_diff;
+int kr_diff_create(kr_problem *fwd, kr_problem *adj, int32_t iterations, int32_t checkpoint_every,
+                   int32_t num_fields, double source_strength, kr_diff **out);
+int kr_diff_destroy(kr_diff *diff);
+int kr_diff_num_exits(kr_diff *diff, int32_t *num_exits);
+int kr_diff_exit(kr_diff *diff, int32_t exit, int32_t *sdom, int32_t *axis, int64_t *num_elem);
+int kr_diff_set_params(kr_diff *diff, int32_t field, const double *const *sigt, const double *const *s);
+int kr_diff_forward(kr_diff *diff, int32_t first, int32_t num, double *const *phi_k, double *const *faces_out);
+int kr_diff_backward(kr_diff *diff, int32_t first, int32_t num, const double *const *phibar,
+                     const double *const *facebar, double *const *grad_sigt, double *const *grad_s);
 
 /* ---- autotuner --------------------------------------------------------- */
 /* No counterpart in the reference.  Times the pipelined sweep (the dominant

@@ -12,34 +12,18 @@ made explicit:
 
 with sigt(g,z) the sigma_t field (sigt_zonal) and s(g,z) the group-diagonal
 n = 0 scattering cross section, both one array per PR chunk in sigt_zonal
-layout.  Since psi_{n+1} depends on psi_n only through phi_n, a checkpoint is
-the (small) phi_n.
-
-Backward, for a cotangent phibar_K of phi_K:
-
-    psibar_K = L^T phibar_K
-    for n = K-1 .. 0:   (phi_n and psi_{n+1} recomputed: the phi of a whole
-                         checkpoint interval once from its checkpoint, then
-                         psi_{n+1} from phi_n)
-        rhsbar_n     = H^-T psibar_{n+1}                 adjoint sweep
-                       (n = K-1: with the cotangent of the last sweep's
-                       exiting boundary faces as adjoint inflow)
-        d sigt      += -sum_{d} rhsbar_n * psi_{n+1}
-        phi_outbar_n = (L+)^T rhsbar_n
-        d s         += phi_outbar_n(0) * phi_n(0)
-        phibar_n     = S^T phi_outbar_n
-        psibar_n     = L^T phibar_n
-
-All of it runs in the library's HIP kernels through the C-ABI; this module only
-sequences the calls and owns the device buffers.  The adjoint variables live
-in a second problem with the same inputs (each adjoint in the field of its
-primal variable), so no field is copied in the backward chain.
+layout.  The forward pass, the phi checkpoints, the recomputation between
+them and the adjoint chain all run inside the library's C-ABI pair
+kr_diff_forward / kr_diff_backward (include/kripke_amd.h); this module owns
+the two problems the pair runs on and passes device addresses.
 """
 import ctypes
 
 import numpy as np
 
 from .capi import HostProblem, check, lib
+
+P = ctypes.c_void_p
 
 
 class DeviceBuffer:
@@ -64,81 +48,75 @@ def _fill(ctx, dst, n, v):
     check(lib().kr_kconst(ctx.h, ctypes.c_void_p(dst), int(n), float(v)), "kr_kconst")
 
 
+def _ptr_array(ptrs):
+    """Host array of device addresses (None entries stay NULL); None -> NULL."""
+    if ptrs is None:
+        return None
+    return (P * len(ptrs))(*[p if p else None for p in ptrs])
+
+
 PLANES = ("i_plane", "j_plane", "k_plane")
 
 
 class DifferentiableSolver:
     """K source iterations as a function of (sigt, s) with phi checkpoints
-    every `checkpoint_every` iterations."""
+    every `checkpoint_every` iterations, for `fields` independent parameter
+    sets (kr_diff)."""
 
-    def __init__(self, ctx, inp, iterations, checkpoint_every=2):
-        if iterations < 1 or checkpoint_every < 1:
-            raise ValueError("iterations and checkpoint_every must be >= 1")
-        self.ctx, self.K, self.ck = ctx, int(iterations), int(checkpoint_every)
+    def __init__(self, ctx, inp, iterations, checkpoint_every=2, fields=1):
+        if iterations < 1 or checkpoint_every < 1 or fields < 1:
+            raise ValueError("iterations, checkpoint_every and fields must be >= 1")
+        self.ctx, self.K, self.ck, self.F = ctx, int(iterations), int(checkpoint_every), int(fields)
         self.fwd = HostProblem(inp, ctx)
         self.adj = HostProblem(inp, ctx)
         self.kf, self.ka = self.fwd.kernels(), self.adj.kernels()
         self.npr, self.nsdom = self.fwd.num_pr, self.fwd.num_sdoms
-        self.sdoms = list(range(self.nsdom))
         # per PR chunk: element counts of the sigma_t and phi chunks
         self.nparam = [self.fwd.device_ptr("sigt_zonal", c)[1] for c in range(self.npr)]
         self.nphi = [self.fwd.device_ptr("phi", c)[1] for c in range(self.npr)]
-        self._s = [DeviceBuffer(ctx, n) for n in self.nparam]        # s(g,z) actually used
-        self._gt = [DeviceBuffer(ctx, n) for n in self.nparam]       # gradient w.r.t. sigt
-        self._gs = [DeviceBuffer(ctx, n) for n in self.nparam]       # gradient w.r.t. s
-        nck = (self.K + self.ck - 1) // self.ck
-        self._ckpt = [[DeviceBuffer(ctx, n) for n in self.nphi] for _ in range(nck)]
-        # phi of the other iterations of one checkpoint interval, recomputed once per interval in backward()
-        self._blk = [[DeviceBuffer(ctx, n) for n in self.nphi] for _ in range(self.ck - 1)]
-        self.forward_done = False
+        self.h = P()
+        check(lib().kr_diff_create(self.kf.h, self.ka.h, self.K, self.ck, self.F, float(inp.source_strength),
+                                   ctypes.byref(self.h)), "kr_diff_create")
+        ne = ctypes.c_int32()
+        check(lib().kr_diff_num_exits(self.h, ctypes.byref(ne)), "kr_diff_num_exits")
+        self.exits = []  # (sdom, plane name, element count), kr_diff's exit order
+        for e in range(ne.value):
+            sd, ax, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+            check(lib().kr_diff_exit(self.h, e, ctypes.byref(sd), ctypes.byref(ax), ctypes.byref(n)), "kr_diff_exit")
+            self.exits.append((sd.value, PLANES[ax.value], n.value))
+        self._gt = [DeviceBuffer(ctx, n) for n in self.nparam]  # gradients of backward()
+        self._gs = [DeviceBuffer(ctx, n) for n in self.nparam]
+        ctx._children.add(self)  # released before its context
 
     # ---- parameters ------------------------------------------------------
-    def set_params(self, sigt_ptrs, s_ptrs):
-        """Device arrays (one per PR chunk, sigt_zonal layout) of sigma_t and s:
-        copied into both problems' sigt_zonal and into the solver's s."""
-        for c in range(self.npr):
-            for hp in (self.fwd, self.adj):
-                _copy(self.ctx, hp.device_ptr("sigt_zonal", c)[0], sigt_ptrs[c], self.nparam[c])
-            _copy(self.ctx, self._s[c].ptr.value, s_ptrs[c], self.nparam[c])
-        self.forward_done = False
+    def set_params(self, sigt_ptrs, s_ptrs, field=0):
+        """Device arrays (one per PR chunk, sigt_zonal layout) of sigma_t and
+        s for one field; copied before the call returns."""
+        check(lib().kr_diff_set_params(self.h, int(field), _ptr_array(sigt_ptrs), _ptr_array(s_ptrs)),
+              "kr_diff_set_params")
 
     def param_ptrs(self, which):
-        """Device addresses of the problem-held parameters ('sigt', 's') or the
-        gradients ('grad_sigt', 'grad_s')."""
+        """Device addresses of the forward problem's sigma_t ('sigt': the
+        generated field until a forward pass writes a field's own) or of the
+        gradients of the last backward() ('grad_sigt', 'grad_s')."""
         if which == "sigt":
             return [self.fwd.device_ptr("sigt_zonal", c)[0] for c in range(self.npr)]
-        bufs = {"s": self._s, "grad_sigt": self._gt, "grad_s": self._gs}[which]
+        bufs = {"grad_sigt": self._gt, "grad_s": self._gs}[which]
         return [b.ptr.value for b in bufs]
 
     # ---- forward ---------------------------------------------------------
-    def _tail(self):
-        """From fwd.phi = phi_n: phi_out, rhs and psi_{n+1} (phi left as phi_n)."""
-        self.kf.scattering_zonal(self.param_ptrs("s"))
-        self.kf.source(self.fwd.inp.source_strength)
-        self.kf.lplustimes(accumulate=False)
-        self.kf.sweep_solver(self.sdoms)
+    def forward(self, field=0):
+        """Runs the K iterations of one field from psi = 0; phi_K is left in the
+        forward problem's phi (phi_ptrs()) and the last sweep's exiting faces
+        in its plane chunks (exit_faces())."""
+        self.forward_fields(field, 1)
 
-    def _step(self):
-        """phi_n -> phi_{n+1} in fwd.phi."""
-        self._tail()
-        self.kf.ltimes(accumulate=False)
-
-    def _set_phi(self, bufs):
-        for c in range(self.npr):
-            _copy(self.ctx, self.fwd.device_ptr("phi", c)[0], bufs[c].ptr.value, self.nphi[c])
-
-    def forward(self):
-        """Runs the K iterations from psi = 0; phi_K is left in the forward
-        problem's phi (phi_ptrs())."""
-        for c in range(self.npr):
-            _fill(self.ctx, self.fwd.device_ptr("phi", c)[0], self.nphi[c], 0.0)  # phi_0 = L 0
-        for n in range(self.K):
-            if n % self.ck == 0:
-                for c in range(self.npr):
-                    _copy(self.ctx, self._ckpt[n // self.ck][c].ptr.value, self.fwd.device_ptr("phi", c)[0],
-                          self.nphi[c])
-            self._step()
-        self.forward_done = True
+    def forward_fields(self, first, num, phi_k=None, faces=None):
+        """Fields [first, first+num) in one call.  phi_k: num*num_pr device
+        addresses (field-major) receiving each field's phi_K; faces:
+        num*len(exits) addresses receiving its exiting faces (either None)."""
+        check(lib().kr_diff_forward(self.h, int(first), int(num), _ptr_array(phi_k), _ptr_array(faces)),
+              "kr_diff_forward")
 
     def phi_ptrs(self, problem="fwd"):
         hp = self.fwd if problem == "fwd" else self.adj
@@ -148,75 +126,48 @@ class DifferentiableSolver:
         """(sdom, plane name, device address, count) of every plane chunk of
         the forward problem that holds faces leaving the domain; after
         forward() they hold the last sweep's exiting angular flux."""
-        out = []
-        for s in self.sdoms:
-            for a, leaves in enumerate(self.kf.sdom_exits(s)):
-                if leaves:
-                    ptr, n = self.fwd.device_ptr(PLANES[a], s)
-                    out.append((s, PLANES[a], ptr, n))
-        return out
+        return [(s, name, self.fwd.device_ptr(name, s)[0], n) for s, name, n in self.exits]
 
     # ---- backward --------------------------------------------------------
-    def backward(self, phibar_ptrs=None, facebar=None):
+    def backward(self, phibar_ptrs=None, facebar=None, field=0):
         """Gradient of <phibar, phi_K> + <facebar, exiting faces of the last
-        sweep> w.r.t. sigt and s, left in param_ptrs('grad_sigt') and
-        param_ptrs('grad_s').  phibar_ptrs: one device array per PR chunk (None:
-        zero); facebar: {(sdom, plane name): device array} over exit_faces()."""
-        if not self.forward_done:
-            raise RuntimeError("backward needs the forward pass of the same parameters")
-        exits = {(s, name): n for s, name, _, n in self.exit_faces()}
-        for key in (facebar or {}):
-            if key not in exits:
-                raise ValueError("facebar names a plane chunk that does not leave the domain: %r" % (key,))
-        for c in range(self.npr):
-            if phibar_ptrs is None:
-                _fill(self.ctx, self.adj.device_ptr("phi", c)[0], self.nphi[c], 0.0)
-            else:
-                _copy(self.ctx, self.adj.device_ptr("phi", c)[0], phibar_ptrs[c], self.nphi[c])
-            _fill(self.ctx, self._gt[c].ptr.value, self.nparam[c], 0.0)
-            _fill(self.ctx, self._gs[c].ptr.value, self.nparam[c], 0.0)
-        self.ka.ltimes_adjoint()                                  # psibar_K
-        gt, gs = self.param_ptrs("grad_sigt"), self.param_ptrs("grad_s")
-        for n in range(self.K - 1, -1, -1):
-            b, c0 = n // self.ck, (n // self.ck) * self.ck
-            if n == min(self.K, c0 + self.ck) - 1:
-                # last iteration of an interval: recompute phi_{c0+1} .. phi_n once from the checkpoint
-                self._set_phi(self._ckpt[b])
-                for j in range(c0, n):
-                    self._step()
-                    for c in range(self.npr):
-                        _copy(self.ctx, self._blk[j - c0][c].ptr.value, self.fwd.device_ptr("phi", c)[0],
-                              self.nphi[c])
-            else:
-                self._set_phi(self._ckpt[b] if n == c0 else self._blk[n - c0 - 1])  # phi_n
-            self._tail()                                          # psi_{n+1}; phi still phi_n
-            if n == self.K - 1 and facebar:
-                for (s, name), n_el in exits.items():             # adjoint inflow: the faces' cotangent
-                    dst = self.adj.device_ptr(name, s)[0]
-                    if (s, name) in facebar:
-                        _copy(self.ctx, dst, facebar[(s, name)], n_el)
-                    else:
-                        _fill(self.ctx, dst, n_el, 0.0)
-                self.ka.sweep_solver_adjoint(seeded=True)         # rhsbar_{K-1}
-            else:
-                self.ka.sweep_solver_adjoint()                    # rhsbar_n
-            self.kf.grad_sigt(self.ka, gt, accumulate=True)
-            self.ka.lplustimes_adjoint()                          # phi_outbar_n
-            self.kf.grad_sigs_zonal(self.ka, gs, accumulate=True)
-            self.ka.scattering_zonal(self.param_ptrs("s"), adjoint=True)  # phibar_n
-            self.ka.ltimes_adjoint()                              # psibar_n
+        sweep> w.r.t. sigt and s of one field, left in param_ptrs('grad_sigt')
+        and param_ptrs('grad_s').  phibar_ptrs: one device array per PR chunk
+        (None: zero); facebar: {(sdom, plane name): device array} over
+        exit_faces() (None: no face term)."""
+        faces = None
+        if facebar is not None:
+            keys = [(s, name) for s, name, _ in self.exits]
+            for key in facebar:
+                if key not in keys:
+                    raise ValueError("facebar names a plane chunk that does not leave the domain: %r" % (key,))
+            faces = [facebar.get(key) for key in keys]
+        self.backward_fields(field, 1, phibar_ptrs, faces, self.param_ptrs("grad_sigt"), self.param_ptrs("grad_s"))
+
+    def backward_fields(self, first, num, phibar=None, faces=None, grad_sigt=None, grad_s=None):
+        """Fields [first, first+num) in one call (flat, field-major address
+        lists as forward_fields)."""
+        check(lib().kr_diff_backward(self.h, int(first), int(num), _ptr_array(phibar), _ptr_array(faces),
+                                     _ptr_array(grad_sigt), _ptr_array(grad_s)), "kr_diff_backward")
 
     def synchronize(self):
         self.ctx.synchronize()
 
     def close(self):
-        for b in self._s + self._gt + self._gs:
-            b.free()
-        for ck in self._ckpt + self._blk:
-            for b in ck:
+        if self.h:
+            lib().kr_diff_destroy(self.h)
+            self.h = None
+        if self.ctx.h:
+            for b in self._gt + self._gs:
                 b.free()
         self.fwd.close()
         self.adj.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def chunk_get(ctx, ptr, n):

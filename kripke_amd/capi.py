@@ -117,6 +117,16 @@ SIGNATURES = [
     ("kr_sdom_exits", ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     ("kr_grad_sigt", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
     ("kr_grad_sigs_zonal", ctypes.c_int, [P, P, ctypes.POINTER(P), ctypes.c_int32]),
+    ("kr_diff_create", ctypes.c_int, [P, P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                      ctypes.POINTER(P)]),
+    ("kr_diff_destroy", ctypes.c_int, [P]),
+    ("kr_diff_num_exits", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32)]),
+    ("kr_diff_exit", ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_int64)]),
+    ("kr_diff_set_params", ctypes.c_int, [P, ctypes.c_int32, ctypes.POINTER(P), ctypes.POINTER(P)]),
+    ("kr_diff_forward", ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(P), ctypes.POINTER(P)]),
+    ("kr_diff_backward", ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(P), ctypes.POINTER(P),
+                                        ctypes.POINTER(P), ctypes.POINTER(P)]),
     ("kr_problem_autotune", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32)]),
     ("kr_batch_create", ctypes.c_int, [P, ctypes.POINTER(P), DP, ctypes.c_int32, ctypes.POINTER(P)]),
     ("kr_batch_destroy", ctypes.c_int, [P]),

@@ -202,7 +202,11 @@ double Kripke::SteadyStateIteration(DataStore &ds, int mode, bool block_jacobi) 
       SweepSolver(ds, all, true, true);
       return Kernel::population(ds);
     }
-    double part = 0.0;  // SweepSolver + population in one pass over psi
+    // SweepSolver + population in one pass over psi.  psi is the
+    // reference's; the plane fields are not (kr_sweep_solver writes faces
+    // straight downwind, so only the domain's downwind-boundary chunks end
+    // with their own outgoing faces), and nothing here reads them afterwards
+    double part = 0.0;
     {
       KRIPKE_TIMER(ds, SweepSolver);
       check(kr_sweep_solver_population(kp(ds), all.data(), (int32_t)all.size(), &part), "SweepSolver");

@@ -357,3 +357,119 @@ def test_torch_gradcheck():
                        timeout=300)
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout[-2000:]
+
+
+# --------------------------------------------------------------------------
+# the C-ABI forward/backward pair (kr_diff) against the same chain sequenced
+# call by call from Python
+# --------------------------------------------------------------------------
+def _sequenced(ctx, layout, iterations, ck):
+    import kripke_amd as ka
+    from adjoint_sequence import SequencedSolver
+    inp = ka.input_vars(**kwargs_for(layout, zones=(6, 6, 6), zset=(2, 1, 1), groups=2, quad=(4, 2), legendre=1))
+    return SequencedSolver(ctx, inp, iterations, checkpoint_every=ck)
+
+
+@pytest.mark.parametrize("layout,faces", [("DGZ", False), ("GZD", False), ("DZG", True), ("ZGD", True)])
+def test_diff_pair_matches_python_sequence(ctx, layout, faces):
+    """kr_diff_forward/kr_diff_backward give the phi_K, exiting faces and
+    gradients of the Python-sequenced chain bit for bit (phi and face
+    cotangents; 5 iterations, checkpoints every 2)."""
+    from kripke_amd.adjoint import DeviceBuffer, chunk_get, chunk_set
+    sv = _solver(ctx, layout, iterations=5, ck=2)
+    sq = _sequenced(ctx, layout, iterations=5, ck=2)
+    sigt, s, bt, bs = _params(sv, ctx)
+    rng = np.random.default_rng(51)
+    w = [rng.standard_normal(n) for n in sv.nphi]
+    wbufs = [DeviceBuffer(ctx, n) for n in sv.nphi]
+    for b, v in zip(wbufs, w):
+        chunk_set(ctx, b.ptr.value, v)
+    out = []
+    for solver in (sv, sq):
+        solver.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs])
+        solver.forward()
+        phi = [chunk_get(ctx, p, n) for p, n in zip(solver.phi_ptrs(), solver.nphi)]
+        ex = solver.exit_faces()
+        fvals = [chunk_get(ctx, p, n) for _, _, p, n in ex]
+        fb = None
+        fbufs = []
+        if faces:
+            fb = {}
+            for (sd, nm, _, n), v in zip(ex, fvals):
+                b = DeviceBuffer(ctx, n)
+                chunk_set(ctx, b.ptr.value, np.sin(v))
+                fb[(sd, nm)] = b.ptr.value
+                fbufs.append(b)
+        solver.backward([b.ptr.value for b in wbufs], fb)
+        gt = [chunk_get(ctx, p, n) for p, n in zip(solver.param_ptrs("grad_sigt"), solver.nparam)]
+        gs = [chunk_get(ctx, p, n) for p, n in zip(solver.param_ptrs("grad_s"), solver.nparam)]
+        out.append((phi, fvals, gt, gs, [(sd, nm) for sd, nm, _, _ in ex]))
+        for b in fbufs:
+            b.free()
+    a, b = out
+    assert a[4] == b[4], "exit lists differ"
+    for x, y in zip(a[0] + a[1] + a[2] + a[3], b[0] + b[1] + b[2] + b[3]):
+        assert np.array_equal(x, y)
+    assert any(np.any(g != 0.0) for g in a[2] + a[3])
+    for buf in bt + bs + wbufs:
+        buf.free()
+    sv.close()
+    sq.close()
+
+
+def test_diff_fields_in_one_call(ctx):
+    """Three fields through one kr_diff_forward and one kr_diff_backward call
+    equal the same fields run one at a time (phi_K, exiting faces,
+    gradients; bit for bit); backward of a field whose parameters changed
+    after its forward pass is refused."""
+    import kripke_amd as ka
+    from kripke_amd.adjoint import DeviceBuffer, DifferentiableSolver, chunk_get, chunk_set
+    inp = ka.input_vars(**kwargs_for("DGZ", zones=(6, 6, 6), zset=(1, 2, 1), groups=2, quad=(4, 2), legendre=1))
+    F = 3
+    sv = DifferentiableSolver(ctx, inp, 4, checkpoint_every=3, fields=F)
+    sigt = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("sigt"), sv.nparam)]
+    rng = np.random.default_rng(52)
+    keep = []
+    for f in range(F):
+        bt = [DeviceBuffer(ctx, n) for n in sv.nparam]
+        bs = [DeviceBuffer(ctx, n) for n in sv.nparam]
+        for b, c, v in zip(bt, bs, sigt):
+            pert = v * (1.0 + 0.3 * rng.uniform(-1, 1, v.size))
+            chunk_set(ctx, b.ptr.value, pert)
+            chunk_set(ctx, c.ptr.value, 0.4 * pert)
+        sv.set_params([b.ptr.value for b in bt], [b.ptr.value for b in bs], field=f)
+        keep += bt + bs
+    ne, npr = len(sv.exits), sv.npr
+    phik = [DeviceBuffer(ctx, n) for _ in range(F) for n in sv.nphi]
+    faces = [DeviceBuffer(ctx, n) for _ in range(F) for _, _, n in sv.exits]
+    wb = [DeviceBuffer(ctx, n) for _ in range(F) for n in sv.nphi]
+    fb = [DeviceBuffer(ctx, n) for _ in range(F) for _, _, n in sv.exits]
+    for b in wb + fb:
+        chunk_set(ctx, b.ptr.value, rng.standard_normal(b.n))
+    gt = [DeviceBuffer(ctx, n) for _ in range(F) for n in sv.nparam]
+    gs = [DeviceBuffer(ctx, n) for _ in range(F) for n in sv.nparam]
+    sv.forward_fields(0, F, [b.ptr.value for b in phik], [b.ptr.value for b in faces])
+    sv.backward_fields(0, F, [b.ptr.value for b in wb], [b.ptr.value for b in fb], [b.ptr.value for b in gt],
+                       [b.ptr.value for b in gs])
+    get = lambda bufs: [chunk_get(ctx, b.ptr.value, b.n) for b in bufs]  # noqa: E731
+    all_phi, all_faces, all_gt, all_gs = get(phik), get(faces), get(gt), get(gs)
+    for f in (2, 0, 1):
+        sv.forward(f)
+        phi = [chunk_get(ctx, p, n) for p, n in zip(sv.phi_ptrs(), sv.nphi)]
+        fv = [chunk_get(ctx, p, n) for _, _, p, n in sv.exit_faces()]
+        fbar = {(sd, nm): b.ptr.value for (sd, nm, _), b in zip(sv.exits, fb[f * ne:(f + 1) * ne])}
+        sv.backward([b.ptr.value for b in wb[f * npr:(f + 1) * npr]], fbar, field=f)
+        g1 = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("grad_sigt"), sv.nparam)]
+        g2 = [chunk_get(ctx, p, n) for p, n in zip(sv.param_ptrs("grad_s"), sv.nparam)]
+        for x, y in zip(phi + fv + g1 + g2, all_phi[f * npr:(f + 1) * npr] + all_faces[f * ne:(f + 1) * ne] +
+                        all_gt[f * npr:(f + 1) * npr] + all_gs[f * npr:(f + 1) * npr]):
+            assert np.array_equal(x, y), "field %d differs from its own run" % f
+    assert not np.array_equal(all_gt[0], all_gt[npr])  # the fields really differ
+    sv.set_params([b.ptr.value for b in keep[0:npr]], [b.ptr.value for b in keep[npr:2 * npr]], field=1)
+    with pytest.raises(ka.KripkeError):
+        sv.backward(field=1)
+    with pytest.raises(ka.KripkeError):
+        sv.forward_fields(1, F)  # range beyond the fields
+    for b in keep + phik + faces + wb + fb + gt + gs:
+        b.free()
+    sv.close()
