@@ -169,6 +169,20 @@ int kr_sweep_solver_bj(kr_problem *prob, const int32_t *sdom_list, int32_t num);
  * kr_sweep_solver; the population agrees with kr_population to rounding
  * (deterministic, fixed-order sums).  Blocks until the value is on the host. */
 int kr_sweep_solver_population(kr_problem *prob, const int32_t *sdom_list, int32_t num, double *population);
+/* No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  kr_sweep_solver_population with the fused population's
+ * per-work-unit partials and their sum in a static device array of
+ * KR_NOALLOC_SCRATCH_DOUBLES doubles (1 MiB, held by each device the
+ * library runs on until the process ends) instead of buffers the problem
+ * allocates on first use: the call allocates nothing and records no timers.
+ * Same psi and population as kr_sweep_solver_population, bit for bit.
+ * Problems whose pipelined sweep has KR_NOALLOC_SCRATCH_DOUBLES - 1 work
+ * units or more are refused with KR_ERR_UNSUPPORTED.  All calls on one
+ * device share its array: calls that overlap on one device (several
+ * contexts or threads) race.  Blocks until the value is on the host. */
+#define KR_NOALLOC_SCRATCH_DOUBLES 131072
+int kr_sweep_solver_population_noalloc(kr_problem *prob, const int32_t *sdom_list, int32_t num, double *population);
 /* Kernel::population (src/Kripke/Kernel/Population.cpp:74-101):
  * sum of w(d)*psi(d,g,z)*volume(z); deterministic fixed-order tree sum.
  * Blocks until the value is on the host. */

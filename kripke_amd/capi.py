@@ -107,6 +107,7 @@ SIGNATURES = [
     ("kr_sweep_solver_bj", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     ("kr_sweep_solver_population", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, DP]),
     ("kr_population", ctypes.c_int, [P, DP]),
+    ("kr_sweep_solver_population_noalloc", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32, DP]),
     ("kr_ltimes_adjoint", ctypes.c_int, [P]),
     ("kr_lplustimes_adjoint", ctypes.c_int, [P]),
     ("kr_scattering_adjoint", ctypes.c_int, [P, ctypes.c_int32]),
@@ -271,6 +272,15 @@ class KernelProblem:
         v = ctypes.c_double()
         check(lib().kr_sweep_solver_population(self.h, arr, len(sdoms), ctypes.byref(v)),
               "kr_sweep_solver_population")
+        return v.value
+
+    def sweep_solver_population_noalloc(self, sdoms):
+        """kr_sweep_solver_population_noalloc: the same with the partials in
+        the device's static scratch array (no device allocation)."""
+        arr = (ctypes.c_int32 * len(sdoms))(*sdoms)
+        v = ctypes.c_double()
+        check(lib().kr_sweep_solver_population_noalloc(self.h, arr, len(sdoms), ctypes.byref(v)),
+              "kr_sweep_solver_population_noalloc")
         return v.value
 
     def population(self):

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,35 @@ void set_error(const std::string &msg) { g_last_error = msg; }
 }  // namespace kr
 
 using namespace kr;
+
+// Scratch of kr_sweep_solver_population_noalloc, the entry point that
+// allocates nothing: the fused population's per-work-unit partials followed
+// by their sum.  Every device holds its own copy for the life of the process;
+// host code reaches it only through the device address hipGetSymbolAddress
+// returns on that device (kr_context::noalloc_scratch), never through the
+// array's name.
+static __device__ __align__(256) double scratch[KR_NOALLOC_SCRATCH_DOUBLES];
+
+// Per-device addresses of `scratch`, looked up once per device at context
+// creation with that device current.
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_addr;
+
+static int noalloc_scratch_address(int device, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return KR_ERR_HIP;
+    }
+    it = g_scratch_addr.emplace(device, (double *)ptr).first;
+  }
+  *out = it->second;
+  return KR_OK;
+}
 
 // A batch whose member problem is destroyed keeps its handle valid (so that
 // kr_batch_destroy still works) but refuses kr_batch_iterate.
@@ -272,6 +302,13 @@ int kr_context_create(int32_t device, kr_context **out) {
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     c->lds_max = (size_t)lds;
+  {
+    const int rc = noalloc_scratch_address(device, &c->noalloc_scratch);
+    if (rc) {
+      delete c;
+      return rc;
+    }
+  }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete c;
@@ -1675,6 +1712,43 @@ int kr_batch_iterate(kr_batch *b, int32_t niter, double *populations) {
   KR_HIP_CHECK(hipMemcpyAsync(populations, b->d_pop, sizeof(double) * b->probs.size() * niter,
                               hipMemcpyDeviceToHost, s0));
   KR_HIP_CHECK(hipStreamSynchronize(s0));
+  return KR_OK;
+}
+
+// kr_sweep_solver_population with the fused population's partials and sum in
+// the device's static scratch array instead of problem buffers allocated on
+// first use: no allocation of any kind on the device, and no timers (their
+// events are created per call).  The subdomain list is checked in place.
+int kr_sweep_solver_population_noalloc(kr_problem *p, const int32_t *list, int32_t num, double *population) {
+  if (!p || !list || !population || num != p->nsdom)
+    return fail(KR_ERR_INVALID, "kr_sweep_solver_population_noalloc: the list must name every subdomain once");
+  for (int32_t i = 0; i < num; ++i) {
+    if (list[i] < 0 || list[i] >= p->nsdom)
+      return fail(KR_ERR_INVALID, "kr_sweep_solver_population_noalloc: the list must name every subdomain once");
+    for (int32_t j = 0; j < i; ++j)
+      if (list[j] == list[i])
+        return fail(KR_ERR_INVALID, "kr_sweep_solver_population_noalloc: the list must name every subdomain once");
+  }
+  double *buf = p->ctx->noalloc_scratch;
+  if (!buf) return fail(KR_ERR_HIP, "kr_sweep_solver_population_noalloc: no scratch address for this device");
+  const int64_t n = p->pipe_level_begin.empty() ? 0 : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
+  if (n + 1 > KR_NOALLOC_SCRATCH_DOUBLES)
+    return fail(KR_ERR_UNSUPPORTED, "kr_sweep_solver_population_noalloc: " + std::to_string(n) +
+                                        " sweep work units need more than the " +
+                                        std::to_string(KR_NOALLOC_SCRATCH_DOUBLES) + "-double scratch");
+  int rc = launch_sweep_coef(p->ctx->stream, p->d_sd, 0, p->nsdom);
+  if (rc) return rc;
+  if ((rc = launch_vol_info(p->ctx->stream, p->d_r, p->nr))) return rc;
+  // every partial is written (not accumulated) by its unit, so nothing in the
+  // scratch needs clearing
+  for (size_t lv = 0; lv < p->pipe_level_begin.size(); ++lv) {
+    rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
+                      p->pipe_level_count[lv], p->shape_pipe, buf + p->pipe_level_begin[lv]);
+    if (rc) return rc;
+  }
+  if ((rc = launch_sum(p->ctx->stream, buf, n, buf + n))) return rc;
+  KR_HIP_CHECK(hipMemcpyAsync(population, buf + n, sizeof(double), hipMemcpyDeviceToHost, p->ctx->stream));
+  KR_HIP_CHECK(hipStreamSynchronize(p->ctx->stream));
   return KR_OK;
 }
 

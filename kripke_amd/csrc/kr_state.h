@@ -25,6 +25,7 @@ struct kr_context {
   int device;
   hipStream_t stream;
   size_t lds_max = 65536;  // LDS per workgroup (hipDeviceAttributeMaxSharedMemoryPerBlock)
+  double *noalloc_scratch = nullptr;  // this device's address of the no-allocation scratch array
   std::map<std::vector<int64_t>, TuneChoice> tuned;  // shape key -> choice
 };
 
