@@ -471,15 +471,26 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     // patch: as many (j,k) columns as fit, subject to the pipeline distances
     // (a patch's downstream neighbour prefetches a face COL_RING steps before
     // it uses it, and must do so at least one step after the face is written).
+    // Among patches of that many columns, the one with the least face traffic
+    // between patches: a face crossing between patches is written to and read
+    // back from the own plane chunk, 16 bytes per pair per patch extent along
+    // that axis, so the cost is 1/pj + 1/pk over the axes cut into patches
+    // (64 columns of 32x32: 8x8 moves 4 B per zone-pair, 32x2 moves 8.5 B).
     int best_pj = 0, best_pk = 0;
+    double best_cost = 0.0;
     for (int pj = largest_divisor_le(nj, cap); pj >= 1; pj = largest_divisor_le(nj, pj - 1)) {
-      const int pk = largest_divisor_le(nk, cap / pj);
-      const int npj = nj / pj;
-      const bool ok_j = (npj == 1) || (ni >= pj + COL_RING);
-      const bool ok_k = (nk / pk == 1) || ((int64_t)npj * ni >= pk + COL_RING);
-      if (ok_j && ok_k && pj * pk > best_pj * best_pk) {
-        best_pj = pj;
-        best_pk = pk;
+      for (int pk = largest_divisor_le(nk, cap / pj); pk >= 1; pk = largest_divisor_le(nk, pk - 1)) {
+        const int npj = nj / pj, npk = nk / pk;
+        const bool ok_j = (npj == 1) || (ni >= pj + COL_RING);
+        const bool ok_k = (npk == 1) || ((int64_t)npj * ni >= pk + COL_RING);
+        const double cost = (npj > 1 ? 1.0 / pj : 0.0) + (npk > 1 ? 1.0 / pk : 0.0);
+        if (ok_j && ok_k &&
+            (pj * pk > best_pj * best_pk || (pj * pk == best_pj * best_pk && cost < best_cost))) {
+          best_pj = pj;
+          best_pk = pk;
+          best_cost = cost;
+        }
+        if (pk == 1) break;
       }
       if (pj == 1) break;
     }
