@@ -291,6 +291,16 @@ int kr_diff_backward(kr_diff *diff, int32_t first, int32_t num, const double *co
  * kr_batch_create).  choice (may be NULL) receives {kernel kind, zones per
  * lane or patch columns, unit size in percent of the default}. */
 int kr_problem_autotune(kr_problem *prob, int32_t *choice);
+/* No counterpart in the reference.  Installs one point of the autotuner's
+ * search space for the pipelined sweep instead of timing them all:
+ * v_max caps the row-scan kernel's zones per lane (zone-fastest nestings,
+ * 0 = no cap) or, in the other nestings, selects the kernel (0 = systolic
+ * pencils, 2 = wave-uniform planes where the shape fits it); cap_div
+ * shrinks the systolic patch or the wave kernel's waves per workgroup;
+ * r_scale_pct scales the pair groups per work unit.  kr_problem_sweep_kind
+ * reports the kernel installed: 0 systolic, 1 row scan, 3 wave-uniform. */
+int kr_problem_set_tuning(kr_problem *prob, int32_t v_max, int32_t cap_div, int32_t r_scale_pct);
+int kr_problem_sweep_kind(kr_problem *prob, int32_t *kind);
 
 /* ---- batched persistent launch ------------------------------------------ */
 /* No counterpart in the reference: many independent problems (each a

@@ -129,6 +129,8 @@ SIGNATURES = [
     ("kr_diff_backward", ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(P), ctypes.POINTER(P),
                                         ctypes.POINTER(P), ctypes.POINTER(P)]),
     ("kr_problem_autotune", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32)]),
+    ("kr_problem_set_tuning", ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    ("kr_problem_sweep_kind", ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int32)]),
     ("kr_batch_create", ctypes.c_int, [P, ctypes.POINTER(P), DP, ctypes.c_int32, ctypes.POINTER(P)]),
     ("kr_batch_destroy", ctypes.c_int, [P]),
     ("kr_batch_iterate", ctypes.c_int, [P, ctypes.c_int32, DP]),
@@ -332,6 +334,14 @@ class KernelProblem:
         c = (ctypes.c_int32 * 3)()
         check(lib().kr_problem_autotune(self.h, c), "kr_problem_autotune")
         return tuple(c)
+
+    def set_tuning(self, v_max=0, cap_div=1, r_scale_pct=100):
+        """Install one autotuner candidate (kr_problem_set_tuning); returns the
+        sweep kernel kind now installed (0 systolic, 1 row scan, 3 wave)."""
+        check(lib().kr_problem_set_tuning(self.h, v_max, cap_div, r_scale_pct), "kr_problem_set_tuning")
+        k = ctypes.c_int32()
+        check(lib().kr_problem_sweep_kind(self.h, ctypes.byref(k)), "kr_problem_sweep_kind")
+        return k.value
 
     def set_timing(self, on=True):
         check(lib().kr_problem_set_timing(self.h, 1 if on else 0), "kr_problem_set_timing")

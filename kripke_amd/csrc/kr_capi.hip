@@ -1054,6 +1054,22 @@ static int install_choice(kr_problem *p, const TuneChoice &c) {
   return KR_OK;
 }
 
+int kr_problem_set_tuning(kr_problem *p, int32_t v_max, int32_t cap_div, int32_t r_scale_pct) {
+  if (!p || v_max < 0 || v_max > 2 || cap_div < 1 || cap_div > 16 || r_scale_pct < 1 || r_scale_pct > 10000)
+    return fail(KR_ERR_INVALID, "kr_problem_set_tuning: bad argument");
+  TuneChoice c;
+  c.v_max = v_max;
+  c.cap_div = cap_div;
+  c.r_scale_pct = r_scale_pct;
+  return install_choice(p, c);
+}
+
+int kr_problem_sweep_kind(kr_problem *p, int32_t *kind) {
+  if (!p || !kind) return fail(KR_ERR_INVALID, "kr_problem_sweep_kind: null argument");
+  *kind = p->shape_pipe.kind;
+  return KR_OK;
+}
+
 int kr_problem_autotune(kr_problem *p, int32_t *choice) {
   if (!p) return fail(KR_ERR_INVALID, "kr_problem_autotune: null problem");
   const std::vector<int64_t> key = tune_key(p);
@@ -1073,12 +1089,17 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
           cand.push_back(c);
         }
       } else {
-        for (int cd : {1, 2}) {
-          TuneChoice c;
-          c.cap_div = cd;
-          c.r_scale_pct = pct;
-          cand.push_back(c);
-        }
+        // the systolic kernel with its largest patch or half of it (v_max 0),
+        // the wave kernel with 16 or 8 waves per workgroup (v_max 2; shapes it
+        // does not cover fall back to the systolic one)
+        for (int v : {0, 2})
+          for (int cd : {1, 2}) {
+            TuneChoice c;
+            c.v_max = v;
+            c.cap_div = cd;
+            c.r_scale_pct = pct;
+            cand.push_back(c);
+          }
       }
     }
     hipEvent_t e0, e1;
@@ -1092,12 +1113,24 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
         if (rc == KR_ERR_UNSUPPORTED) continue;
         break;
       }
+      // timed as the pipelined solve runs it: with the fused population
+      // partials (one per unit) and the volume check before it
+      const int64_t nu = p->pipe_level_begin.empty() ? 0
+                                                     : (int64_t)p->pipe_level_begin.back() + p->pipe_level_count.back();
+      if (nu > p->pop_units_cap) {
+        if (p->d_pop_units) KR_HIP_CHECK(hipFree(p->d_pop_units));
+        p->d_pop_units = nullptr;
+        p->pop_units_cap = 0;
+        KR_HIP_CHECK(hipMalloc((void **)&p->d_pop_units, std::max<int64_t>(nu, 1) * sizeof(double)));
+        p->pop_units_cap = nu;
+      }
+      if ((rc = launch_vol_info(p->ctx->stream, p->d_r, p->nr))) break;
       double ms = 1e300;
       for (int rep = 0; rep < 3 && !rc; ++rep) {  // first run warms up; keep the best of the rest
         hipEventRecord(e0, p->ctx->stream);
         for (size_t lv = 0; lv < p->pipe_level_begin.size() && !rc; ++lv)
           rc = launch_sweep(p->ctx->stream, p->d_sd, p->d_planes_pipe, p->d_units_pipe + p->pipe_level_begin[lv],
-                            p->pipe_level_count[lv], p->shape_pipe);
+                            p->pipe_level_count[lv], p->shape_pipe, p->d_pop_units + p->pipe_level_begin[lv]);
         hipEventRecord(e1, p->ctx->stream);
         hipEventSynchronize(e1);
         float t = 0.f;
@@ -1465,7 +1498,7 @@ static int build_part(kr_batch *b, BatchPart &B) {
         coef = std::max<int64_t>(coef, (int64_t)S.nd * (S.ni + S.nj + S.nk + 1));
       }
       const int nt = sh.kind == 1 ? sh.RG * sh.PK : sh.Q * sh.PJ * sh.PK;
-      const size_t lds = ((sh.kind == 1 ? 2 * (size_t)sh.V : 4) * nt + (size_t)coef) * sizeof(double);
+      const size_t lds = sweep_lds_bytes(sh, coef);
       if (lds > ctx->lds_max || nt > 1024) return false;
       bs.shape = sh;
       bs.coef = coef;

@@ -121,7 +121,8 @@ struct SweepPlanes {
 
 // Per-launch sweep geometry (same for every unit of a launch).
 struct SweepShape {
-  int32_t kind;                 // 0: systolic pencil kernel, 1: row-scan kernel (zone-fastest nestings), 2: none fits
+  int32_t kind;                 // 0: systolic pencil kernel, 1: row-scan kernel (zone-fastest nestings), 2: none fits,
+                                // 3: wave-uniform plane kernel (Q = 64 pairs, PK = waves, NPK = k-patches)
   int32_t Q, PJ, PK, NPJ, NPK;  // lanes per zone column, patch extents, patches per axis
   int32_t RG;                   // row-scan kernel: lanes per row group
   int32_t V;                    // row-scan kernel: consecutive zones per lane

@@ -106,10 +106,21 @@ int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // v_max caps the row-scan kernel's zones per lane (0: no cap); cap_div
 // shrinks the systolic kernel's largest patch by that factor.
 struct SweepTune {
-  int v_max = 0;
-  int cap_div = 1;
+  int v_max = 0;           // row-scan: zones per lane cap; other nestings: 2 = the wave kernel where it fits
+  int cap_div = 1;         // systolic: patch size divisor; wave kernel: waves per workgroup = 16 / cap_div
   size_t lds_max = 65536;  // the device's LDS per workgroup (shapes within 64 KiB are preferred)
 };
+// Wave-uniform plane sweep of the nestings whose fastest index is a group or
+// a direction (kr_sweep_waves.hip): 64 pairs per wave, one k-layer per wave,
+// at most kt_max waves per workgroup.  Returns false when the subdomain does
+// not fit it (pairs not a multiple of 64, ni not 8, 16 or 32).
+bool waves_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, size_t lds_limit,
+                 int kt_max);
+int launch_sweep_waves(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
+                       int32_t num_units, const SweepShape &shape, double *pop, bool adjoint);
+// LDS bytes one sweep workgroup of `shape` takes, with coef_doubles of
+// coefficient records.
+size_t sweep_lds_bytes(const SweepShape &shape, int64_t coef_doubles);
 // aligned16: every psi/rhs/sigma_t/plane chunk starts on a 16-byte boundary.
 SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,
                               bool aligned16, const SweepTune &tune = SweepTune());

@@ -61,6 +61,7 @@ __device__ __forceinline__ void st(Rsrc r, uint32_t off, double v) {
 
 struct ColArgs {
   const double *rhs, *sig, *coef, *w, *vol;
+  const double *vol_info;  // {1 if every zone volume is the same, that volume} (k_vol_info)
   double *psi;
   const double *own[3];   // the subdomain's plane chunks
   double *out[3];         // destination of faces leaving the subdomain
@@ -223,8 +224,12 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     col_step(lc, P, K);
   };
 
-  double rv = 0.0;
-  if (POP) rv = ld<0>(Rvol, cc.vol8);
+  // zone volumes: one value for the whole subdomain (the generator's meshes:
+  // no per-zone load), else one load per zone step
+  const bool vol_uniform = POP && P.vol_info[0] != 0.0;
+  const double vol0 = POP ? P.vol_info[1] : 0.0;
+  double rv = vol0;
+  if (POP && !vol_uniform) rv = ld<0>(Rvol, cc.vol8);
 #pragma unroll
   for (int r = 0; r < RING; ++r) refill(r);
 
@@ -271,9 +276,10 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       if (POP) {
         const double t = (rec[L - 1] * p) * rv;
         pacc += act ? t : 0.0;
+        asm volatile("" : "+v"(pacc));  // summed in its step, not deferred
       }
       col_step(cc, P, K);
-      if (POP) rv = ld<0>(Rvol, cc.vol8);
+      if (POP && !vol_uniform) rv = ld<0>(Rvol, cc.vol8);
       refill(r);
       __syncthreads();
     }
@@ -308,6 +314,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   P.coef = S.coef[0];
   P.w = S.w;
   P.vol = S.volume;
+  P.vol_info = S.vol_info;
   P.psi = ADJ ? S.rhs : S.psi;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
@@ -456,6 +463,12 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   if (zfast && (row_shape(sh, nd, ng, ni, nj, nk, aligned16, tune.v_max, 65536) ||
                 (tune.lds_max > 65536 && row_shape(sh, nd, ng, ni, nj, nk, aligned16, tune.v_max, tune.lds_max))))
     return sh;
+  if (!zfast && tune.v_max == 2) {
+    const int kt_max = 16 / std::max(1, tune.cap_div);
+    if (waves_shape(sh, nd, ng, ni, nj, nk, std::min<size_t>(65536, tune.lds_max), kt_max) ||
+        (tune.lds_max > 65536 && waves_shape(sh, nd, ng, ni, nj, nk, tune.lds_max, kt_max)))
+      return sh;
+  }
   if (zfast) {
     sh.Q = 1;
   } else {
@@ -508,6 +521,13 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   return sh;
 }
 
+size_t sweep_lds_bytes(const SweepShape &shape, int64_t coef_doubles) {
+  if (shape.kind == 1) return (2 * (size_t)shape.RG * shape.PK * shape.V + (size_t)coef_doubles) * sizeof(double);
+  const size_t nt = (size_t)shape.Q * shape.PJ * shape.PK;
+  if (shape.kind == 3) return std::max(2 * nt + (size_t)coef_doubles, nt) * sizeof(double);
+  return (4 * nt + (size_t)coef_doubles) * sizeof(double);
+}
+
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                  int32_t num_units, const SweepShape &shape, double *pop, bool adjoint) {
   if (num_units <= 0) return KR_OK;
@@ -515,6 +535,7 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
     set_error("sweep: no population in the adjoint sweep");
     return KR_ERR_INVALID;
   }
+  if (shape.kind == 3) return launch_sweep_waves(st, sd, planes, units, num_units, shape, pop, adjoint);
   if (shape.kind == 1)
     return launch_sweep_rows(st, sd, planes, units, num_units, shape,
                              (int64_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1), pop, adjoint);

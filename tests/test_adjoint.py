@@ -473,3 +473,29 @@ def test_diff_fields_in_one_call(ctx):
     for b in keep + phik + faces + wb + fb + gt + gs:
         b.free()
     sv.close()
+
+
+@pytest.mark.parametrize("layout", ["DZG", "GZD", "ZDG", "ZGD"])
+@pytest.mark.parametrize("zones,zset,groups,quad", [((16, 8, 32), (2, 1, 1), 32, (4, 4)),
+                                                    ((32, 16, 12), (2, 2, 2), 64, (4, 2))])
+def test_sweep_transpose_wave_kernel(ctx, layout, zones, zset, groups, quad):
+    """The seeded transpose identity with the wave-uniform plane kernel
+    installed (kr_problem_set_tuning): forward and adjoint sweeps both run it."""
+    hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=groups, zones=zones, quad=quad, legendre=1))
+    k = hp.kernels()
+    assert k.set_tuning(v_max=2) == 3
+    rng = np.random.default_rng(48)
+    exits = [(s, PLANES[a]) for s in range(hp.num_sdoms) for a, e in enumerate(k.sdom_exits(s)) if e]
+    x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.sweep_solver(list(range(hp.num_sdoms)))
+    Hx = _get(hp, "psi", hp.num_sdoms)
+    Bx = [hp.get(name, s) for s, name in exits]
+    y = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    fb = []
+    for s, name in exits:
+        v = rng.standard_normal(hp.get(name, s).size)
+        hp.set(name, s, v)
+        fb.append(v)
+    k.sweep_solver_adjoint(seeded=True)
+    Aty = _get(hp, "rhs", hp.num_sdoms)
+    _check_dot((y + fb, Hx + Bx), (Aty, x))

@@ -546,3 +546,85 @@ def test_nesting_invariance(ctx):
                 assert np.array_equal(x, y), layout
     for x, y in zip(got["DGZ"], got["ZDG"]):
         assert rel_normwise(x, y) <= TOL_SCAN
+
+
+# --------------------------------------------------------------------------
+# the wave-uniform plane kernel (an autotuner candidate for the nestings
+# whose fastest index is a group or a direction): installed explicitly
+# --------------------------------------------------------------------------
+WAVE_LAYOUTS = ("DZG", "GZD", "ZDG", "ZGD")
+# pairs per subdomain a multiple of 64, rows of 8/16/32 zones; (16, 8, 32)
+# in 2 zone sets along i: 8 x 8 x 32 subdomains, two k-patches of 16 waves
+WAVE_SHAPES = [dict(zones=(16, 8, 32), zset=(2, 1, 1), groups=32, quad=(4, 4)),
+               dict(zones=(32, 16, 12), zset=(2, 2, 2), groups=64, quad=(4, 2))]
+
+
+def _waves(hp):
+    kind = hp.kernels().set_tuning(v_max=2)
+    assert kind == 3, "the wave kernel does not cover this shape"
+
+
+@pytest.mark.parametrize("layout", WAVE_LAYOUTS)
+@pytest.mark.parametrize("shape", range(len(WAVE_SHAPES)))
+def test_wave_kernel_pipeline(ctx, layout, shape):
+    """Pipelined sweep with the fused population: psi bit for bit against the
+    fma oracle, the population against the oracle's compensated sum, and the
+    plane chunks on the domain's downwind boundary."""
+    kw = kwargs_for(layout, **WAVE_SHAPES[shape])
+    hp, of, orf = _trio(ctx, **kw)
+    _waves(hp)
+    rng = np.random.default_rng(60 + shape)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    pop = hp.kernels().sweep_solver_population(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=True)
+    _, comp = orf.population()
+    assert abs(pop - comp) <= TOL_REF * abs(comp)
+    for t in range(hp.num_sdoms):
+        info = of.sdom(t)
+        for a, name in enumerate(("i_plane", "j_plane", "k_plane")):
+            if info["down%d" % a] < 0:
+                assert np.array_equal(hp.get(name, t), of.get(name, t)), (t, name)
+
+
+@pytest.mark.parametrize("layout", WAVE_LAYOUTS)
+def test_wave_kernel_block_jacobi_and_plain(ctx, layout):
+    """Block Jacobi (every subdomain from zero incoming faces) and the plain
+    pipelined sweep (no population) with the wave kernel, against the fma
+    oracle bit for bit."""
+    kw = kwargs_for(layout, **WAVE_SHAPES[0])
+    hp, of, orf = _trio(ctx, **kw)
+    _waves(hp)
+    rng = np.random.default_rng(64)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    for name in ("i_plane", "j_plane", "k_plane"):
+        seeded_fill(hp, [of, orf], name, hp.num_sdoms, rng)
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)), block_jacobi=True)
+    of.sweep_solver(block_jacobi=True)
+    for name in ("psi", "i_plane", "j_plane", "k_plane"):
+        _assert_chunks(hp, of, None, name, hp.num_sdoms, exact=True)
+    seeded_fill(hp, [of], "rhs", hp.num_sdoms, rng, offset=1.0)
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    _assert_chunks(hp, of, None, "psi", hp.num_sdoms, exact=True)
+
+
+@pytest.mark.parametrize("layout", WAVE_LAYOUTS)
+def test_wave_kernel_iterations(ctx, layout):
+    """Three on-device source iterations with the wave kernel: phi and the
+    populations equal those of the default (systolic) kernel bit for bit,
+    and the populations agree with the reference-arithmetic oracle."""
+    kw = kwargs_for(layout, **WAVE_SHAPES[1])
+    a, b = product_for(ctx, **kw), product_for(ctx, **kw)
+    _waves(b)
+    orf = oracle_for("ref", **kw)
+    for x in (a, b, orf):
+        x.init_psi()
+    for it in range(3):
+        pa, pb = a.iterate(1), b.iterate(1)
+        _, comp = orf.iterate()
+        assert abs(pb - comp) <= TOL_REF * abs(comp)
+        assert abs(pa - pb) <= 1e-14 * abs(pa)
+        for c in range(a.num_pr):
+            assert np.array_equal(a.get("phi", c), b.get("phi", c)), (it, c)
