@@ -65,13 +65,16 @@ def measured_traffic(layout):
 def run_gpu(layout, steps, warmup):
     """Times `steps` source iterations of the workload in `layout` after
     `warmup` untimed ones; then a breakdown pass with the hipEvent timers,
-    outside the timed region."""
+    outside the timed region.  Setup (outside the timed region) builds the
+    problem and runs the library's sweep autotuner on it
+    (kr_problem_autotune: kernel, patch and work-unit size)."""
     import kripke_amd as ka
     ctx = ka.Context(0)
     t0 = time.time()
     hp = ka.HostProblem(ka.input_vars(layout=layout, **WORKLOAD), ctx)
-    setup_s = time.time() - t0
     k = hp.kernels()
+    tune = k.autotune()
+    setup_s = time.time() - t0
     hp.init_psi()
     for _ in range(warmup):
         hp.iterate(1)
@@ -91,10 +94,13 @@ def run_gpu(layout, steps, warmup):
     k.set_timing(False)
     hp.close()
     ctx.close()
-    return elapsed, setup_s, timers, nb
+    return elapsed, setup_s, timers, nb, tune
 
 
-def nesting_figures(layout, steps, elapsed, timers, nb):
+SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_rows", 3: "k_sweep_waves"}
+
+
+def nesting_figures(layout, steps, elapsed, timers, nb, tune):
     """Whole-iteration FOM, the metric path (sweep + LTimes + LPlusTimes)
     against its algorithmic bytes, and the sweep kernel's own fraction."""
     nz = 64 ** 3
@@ -110,7 +116,8 @@ def nesting_figures(layout, steps, elapsed, timers, nb):
         "metric_path": {"ms_per_iteration": path_t * 1e3, "alg_bytes_per_iteration": path_bytes,
                         "hbm_frac": path_bytes / path_t / HBM_PEAK if path_t > 0 else None,
                         "unknowns_per_s": ab["N"] / path_t if path_t > 0 else None},
-        "sweep_kernel": {"kernel": "k_sweep_rows" if layout in ("DGZ", "GDZ") else "k_sweep",
+        "sweep_kernel": {"kernel": SWEEP_KERNELS.get(tune[0], str(tune[0])),
+                         "autotune_choice": list(tune),
                          "avg_launch_ms": sweep_t / max(sweep_n, 1) * 1e3,
                          "hbm_frac": (ab["sweep"] * nb) / sweep_t / HBM_PEAK if sweep_t > 0 else None},
         "breakdown_ms": {n: timers[n][0] / nb * 1e3 for n in timers},
@@ -407,8 +414,8 @@ def main():
         print(json.dumps({"batched": run_batched()}))
         return
 
-    elapsed, setup_s, timers, nb = run_gpu(args.layout, args.steps, args.warmup)
-    fig = nesting_figures(args.layout, args.steps, elapsed, timers, nb)
+    elapsed, setup_s, timers, nb, tune = run_gpu(args.layout, args.steps, args.warmup)
+    fig = nesting_figures(args.layout, args.steps, elapsed, timers, nb, tune)
     ab = algorithmic_bytes(128, 128, 64 ** 3, 25)
     N = ab["N"]
     sweep_t, sweep_n = timers["k_sweep"]
@@ -428,13 +435,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded 3-material boxes in a 100 cm cube, vacuum boundaries)",
         "config": {"workload": "64^3 zones (2x2x2 zone sets), 128 groups, 128 directions GL 8x16 (8 direction sets), "
-                               "P4 (25 moments), %s nesting, on-device source iteration" % args.layout,
+                               "P4 (25 moments), %s nesting, on-device source iteration, sweep shape autotuned"
+                               % args.layout,
                    "unknowns": N, "nesting": args.layout, "zone_sets": [2, 2, 2], "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": sweep_achieved / 1e9 if sweep_achieved else None,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": sweep_achieved / HBM_PEAK if sweep_achieved else None,
                      "traffic": measured_traffic(args.layout),
                      "kernel": fig["sweep_kernel"]["kernel"],
+                     "autotune_choice": fig["sweep_kernel"]["autotune_choice"],
                      "alg_bytes_per_launch": ab["sweep"] / SWEEP_STAGES,
                      "avg_launch_ms": sweep_t / max(sweep_n, 1) * 1e3},
         "metric_path": fig["metric_path"],
@@ -446,8 +455,8 @@ def main():
         per = {args.layout: fig}
         for lay in NESTINGS:
             if lay != args.layout:
-                e, _, tm, b = run_gpu(lay, NESTING_STEPS, NESTING_WARMUP)
-                per[lay] = nesting_figures(lay, NESTING_STEPS, e, tm, b)
+                e, _, tm, b, tu = run_gpu(lay, NESTING_STEPS, NESTING_WARMUP)
+                per[lay] = nesting_figures(lay, NESTING_STEPS, e, tm, b, tu)
         out["nestings"] = {"steps": NESTING_STEPS, "warmup": NESTING_WARMUP,
                            "figures": {lay: per[lay] for lay in NESTINGS if lay in per}}
     if not args.no_adjoint:
