@@ -295,10 +295,13 @@ int kr_problem_autotune(kr_problem *prob, int32_t *choice);
  * search space for the pipelined sweep instead of timing them all:
  * v_max caps the row-scan kernel's zones per lane (zone-fastest nestings,
  * 0 = no cap) or, in the other nestings, selects the kernel (0 = systolic
- * pencils, 2 = wave-uniform planes where the shape fits it); cap_div
- * shrinks the systolic patch or the wave kernel's waves per workgroup;
+ * pencils, 2 = wave-uniform planes, 3 = systolic pencils with patches over
+ * every j-row and the faces between k-patches in LDS, each where the shape
+ * fits it); cap_div shrinks the systolic patch, the wave kernel's waves per
+ * workgroup or kernel 3's pairs per zone column (16 / cap_div);
  * r_scale_pct scales the pair groups per work unit.  kr_problem_sweep_kind
- * reports the kernel installed: 0 systolic, 1 row scan, 3 wave-uniform. */
+ * reports the kernel installed: 0 systolic, 1 row scan, 3 wave-uniform,
+ * 4 systolic with the LDS k-face ring. */
 int kr_problem_set_tuning(kr_problem *prob, int32_t v_max, int32_t cap_div, int32_t r_scale_pct);
 int kr_problem_sweep_kind(kr_problem *prob, int32_t *kind);
 

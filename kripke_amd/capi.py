@@ -337,7 +337,8 @@ class KernelProblem:
 
     def set_tuning(self, v_max=0, cap_div=1, r_scale_pct=100):
         """Install one autotuner candidate (kr_problem_set_tuning); returns the
-        sweep kernel kind now installed (0 systolic, 1 row scan, 3 wave)."""
+        sweep kernel kind now installed (0 systolic, 1 row scan, 3 wave, 4
+        systolic with the LDS k-face ring)."""
         check(lib().kr_problem_set_tuning(self.h, v_max, cap_div, r_scale_pct), "kr_problem_set_tuning")
         k = ctypes.c_int32()
         check(lib().kr_problem_sweep_kind(self.h, ctypes.byref(k)), "kr_problem_sweep_kind")

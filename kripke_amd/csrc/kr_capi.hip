@@ -1055,7 +1055,7 @@ static int install_choice(kr_problem *p, const TuneChoice &c) {
 }
 
 int kr_problem_set_tuning(kr_problem *p, int32_t v_max, int32_t cap_div, int32_t r_scale_pct) {
-  if (!p || v_max < 0 || v_max > 2 || cap_div < 1 || cap_div > 16 || r_scale_pct < 1 || r_scale_pct > 10000)
+  if (!p || v_max < 0 || v_max > 3 || cap_div < 1 || cap_div > 16 || r_scale_pct < 1 || r_scale_pct > 10000)
     return fail(KR_ERR_INVALID, "kr_problem_set_tuning: bad argument");
   TuneChoice c;
   c.v_max = v_max;
@@ -1066,7 +1066,7 @@ int kr_problem_set_tuning(kr_problem *p, int32_t v_max, int32_t cap_div, int32_t
 
 int kr_problem_sweep_kind(kr_problem *p, int32_t *kind) {
   if (!p || !kind) return fail(KR_ERR_INVALID, "kr_problem_sweep_kind: null argument");
-  *kind = p->shape_pipe.kind;
+  *kind = (p->shape_pipe.kind == 0 && p->shape_pipe.kring) ? 4 : p->shape_pipe.kind;
   return KR_OK;
 }
 
@@ -1090,9 +1090,11 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
         }
       } else {
         // the systolic kernel with its largest patch or half of it (v_max 0),
-        // the wave kernel with 16 or 8 waves per workgroup (v_max 2; shapes it
-        // does not cover fall back to the systolic one)
-        for (int v : {0, 2})
+        // the wave kernel with 16 or 8 waves per workgroup (v_max 2), the
+        // systolic kernel with full-j patches and the LDS k-face ring, at most
+        // 16 or 8 pairs per zone column (v_max 3); shapes a kernel does not
+        // cover fall back to the plain systolic one
+        for (int v : {0, 2, 3})
           for (int cd : {1, 2}) {
             TuneChoice c;
             c.v_max = v;

@@ -106,7 +106,8 @@ int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // v_max caps the row-scan kernel's zones per lane (0: no cap); cap_div
 // shrinks the systolic kernel's largest patch by that factor.
 struct SweepTune {
-  int v_max = 0;           // row-scan: zones per lane cap; other nestings: 2 = the wave kernel where it fits
+  int v_max = 0;           // row-scan: zones per lane cap; other nestings: 2 = the wave kernel, 3 = the systolic
+                           // kernel with full-j patches and an LDS k-face ring, where they fit
   int cap_div = 1;         // systolic: patch size divisor; wave kernel: waves per workgroup = 16 / cap_div
   size_t lds_max = 65536;  // the device's LDS per workgroup (shapes within 64 KiB are preferred)
 };

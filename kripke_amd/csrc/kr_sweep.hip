@@ -157,7 +157,7 @@ __device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const C
   col_patch(c, P, K);
 }
 
-template <bool POP, bool ADJ>
+template <bool POP, bool ADJ, bool KRING>
 __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, double *lds) {
   constexpr int RING = COL_RING;
   const int nt = blockDim.x;
@@ -167,6 +167,11 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   double *fyb = lds;               // [2][nt]
   double *fzb = lds + 2 * nt;      // [2][nt]
   double *co = lds + 4 * nt;       // [nd][L]
+  // KRING (a patch spans every j-row): the faces between k-patches, [ni][PJ][Q]
+  // by the zone's step along its pencil; the top layer of patch pk writes a
+  // slot ni - PK + 1 steps before the bottom layer of patch pk + 1 reads it,
+  // and rewrites it only after that read
+  double *kring = co + nd * L;
   {
     const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], contiguous
     for (int e = tid; e < nd * (ni + nj + nk); e += nt) {
@@ -220,7 +225,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     // patch's outgoing face in the own chunk); z likewise
     if (lc.ii == 0) rx[r] = ld<0>(Rin0, lc.x8);
     if (K.jj == 0) ry[r] = ld<0>(Rown1, (lc.pj > 0 || P.has_in[1]) ? lc.y8 : NOWHERE);
-    if (K.kk == 0) rz[r] = ld<0>(Rown2, (lc.pk > 0 || P.has_in[2]) ? lc.z8 : NOWHERE);
+    if (K.kk == 0) rz[r] = ld<0>(Rown2, ((!KRING && lc.pk > 0) || (lc.pk == 0 && P.has_in[2])) ? lc.z8 : NOWHERE);
     col_step(lc, P, K);
   };
 
@@ -248,7 +253,8 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       const double ck = rec[ni + nj + cc.k];
       if (cc.ii == 0) fx = rx[r];
       const double fy = K.jj == 0 ? ry[r] : fyb[prev * nt + tid - K.Q];
-      const double fzv = K.kk == 0 ? rz[r] : fzb[prev * nt + tid - K.Q * K.PJ];
+      double fzv = K.kk == 0 ? rz[r] : fzb[prev * nt + tid - K.Q * K.PJ];
+      if (KRING && K.kk == 0 && cc.pk > 0) fzv = kring[(cc.ii * K.PJ + K.jj) * K.Q + K.q];
       // adjoint: incoming faces weigh 2, outgoing faces a, b, c (transposed zone relations)
       const double wx = ADJ ? 2.0 : a, wy = ADJ ? 2.0 : bj, wz = ADJ ? 2.0 : ck;
       const double num = __fma_rn(fzv, wz, __fma_rn(fy, wy, __fma_rn(fx, wx, rq[r])));
@@ -268,6 +274,8 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       if (act && K.kk == K.PK - 1) {
         if (cc.pk == K.NPK - 1)
           st<0>(Rout2, cc.z8, fzo);
+        else if (KRING)
+          kring[(cc.ii * K.PJ + K.jj) * K.Q + K.q] = fzo;
         else
           st<0>(Rown2, cc.z8, fzo);
       }
@@ -300,7 +308,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
 }  // namespace
 
 // ADJ: the transposed sweep (reads psi, writes rhs, every axis reversed).
-template <bool POP, bool ADJ>
+template <bool POP, bool ADJ, bool KRING>
 __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes,
                                                 const SweepUnit *__restrict__ units, SweepShape sh,
                                                 double *__restrict__ pop) {
@@ -365,7 +373,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   K.dy8 = (int32_t)(8 * K.istep * S.pf[1]);
   K.dz8 = (int32_t)(8 * K.istep * S.pf[2]);
   K.dvol8 = 8 * K.istep;
-  cols_body<POP, ADJ>(P, K, lds);
+  cols_body<POP, ADJ, KRING>(P, K, lds);
 }
 
 // 2*cos(d)/delta per (direction, zone index), the exact expression of
@@ -476,6 +484,29 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     while (qv < 16 && inner % (qv * 2) == 0) qv *= 2;
     sh.Q = qv;
   }
+  // systolic kernel whose patches span every j-row (tuning point v_max = 3):
+  // the faces between k-patches stay in an LDS ring of [ni][nj][Q] doubles,
+  // so no face leaves the chip inside the subdomain; Q is capped at 16 /
+  // cap_div to fit the ring
+  if (!zfast && tune.v_max == 3) {
+    const int qk = std::max(1, std::min(sh.Q, 16 / std::max(1, tune.cap_div)));
+    const size_t coef = (size_t)nd * (ni + nj + nk + 1);
+    for (int PK = largest_divisor_le(nk, 1024 / std::max(1, qk * nj)); PK >= 1 && qk * nj <= 1024;
+         PK = largest_divisor_le(nk, PK - 1)) {
+      const int nt = qk * nj * PK;
+      const size_t lds = (4 * (size_t)nt + coef + (size_t)ni * nj * qk) * sizeof(double);
+      if ((nk / PK == 1 || ni >= PK + COL_RING) && lds <= tune.lds_max) {
+        sh.Q = qk;
+        sh.PJ = nj;
+        sh.PK = PK;
+        sh.NPJ = 1;
+        sh.NPK = nk / PK;
+        sh.kring = nk / PK > 1 ? 1 : 0;
+        return sh;
+      }
+      if (PK == 1) break;
+    }
+  }
   // LDS: j/k face exchange [4][threads] and the coefficient records [nd][ni+nj+nk+1]
   const size_t coef_lds = (size_t)nd * (ni + nj + nk + 1) * sizeof(double);
   for (int pass = 0; pass < 2; ++pass) {
@@ -525,7 +556,8 @@ size_t sweep_lds_bytes(const SweepShape &shape, int64_t coef_doubles) {
   if (shape.kind == 1) return (2 * (size_t)shape.RG * shape.PK * shape.V + (size_t)coef_doubles) * sizeof(double);
   const size_t nt = (size_t)shape.Q * shape.PJ * shape.PK;
   if (shape.kind == 3) return std::max(2 * nt + (size_t)coef_doubles, nt) * sizeof(double);
-  return (4 * nt + (size_t)coef_doubles) * sizeof(double);
+  const size_t ring = shape.kring ? (size_t)shape.ni * shape.PJ * shape.Q : 0;
+  return (4 * nt + (size_t)coef_doubles + ring) * sizeof(double);
 }
 
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
@@ -540,19 +572,31 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
     return launch_sweep_rows(st, sd, planes, units, num_units, shape,
                              (int64_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1), pop, adjoint);
   const int nt = shape.Q * shape.PJ * shape.PK;
-  const size_t lds = (4 * (size_t)nt + (size_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1)) * sizeof(double);
-  if (lds > 65536) {
-    int rc;
-    if ((rc = allow_lds((const void *)k_sweep<false, true>, lds)) || (rc = allow_lds((const void *)k_sweep<true, false>, lds)) ||
-        (rc = allow_lds((const void *)k_sweep<false, false>, lds)))
-      return rc;
-  }
-  if (adjoint)
-    hipLaunchKernelGGL((k_sweep<false, true>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
-  else if (pop)
-    hipLaunchKernelGGL((k_sweep<true, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
+  const size_t lds = sweep_lds_bytes(shape, (int64_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1));
+#define KR_SWEEP_LAUNCH(KR)                                                                                        \
+  do {                                                                                                             \
+    if (lds > 65536) {                                                                                             \
+      int rc;                                                                                                      \
+      if ((rc = allow_lds((const void *)k_sweep<false, true, KR>, lds)) ||                                         \
+          (rc = allow_lds((const void *)k_sweep<true, false, KR>, lds)) ||                                         \
+          (rc = allow_lds((const void *)k_sweep<false, false, KR>, lds)))                                          \
+        return rc;                                                                                                 \
+    }                                                                                                              \
+    if (adjoint)                                                                                                   \
+      hipLaunchKernelGGL((k_sweep<false, true, KR>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, \
+                         pop);                                                                                     \
+    else if (pop)                                                                                                  \
+      hipLaunchKernelGGL((k_sweep<true, false, KR>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, \
+                         pop);                                                                                     \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_sweep<false, false, KR>), dim3(num_units), dim3(nt), lds, st, sd, planes, units,      \
+                         shape, pop);                                                                              \
+  } while (0)
+  if (shape.kring)
+    KR_SWEEP_LAUNCH(true);
   else
-    hipLaunchKernelGGL((k_sweep<false, false>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, pop);
+    KR_SWEEP_LAUNCH(false);
+#undef KR_SWEEP_LAUNCH
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }

@@ -476,14 +476,17 @@ def test_diff_fields_in_one_call(ctx):
 
 
 @pytest.mark.parametrize("layout", ["DZG", "GZD", "ZDG", "ZGD"])
-@pytest.mark.parametrize("zones,zset,groups,quad", [((16, 8, 32), (2, 1, 1), 32, (4, 4)),
-                                                    ((32, 16, 12), (2, 2, 2), 64, (4, 2))])
-def test_sweep_transpose_wave_kernel(ctx, layout, zones, zset, groups, quad):
-    """The seeded transpose identity with the wave-uniform plane kernel
+@pytest.mark.parametrize("kernel,zones,zset,groups,quad", [((2, 3), (16, 8, 32), (2, 1, 1), 32, (4, 4)),
+                                                           ((2, 3), (32, 16, 12), (2, 2, 2), 64, (4, 2)),
+                                                           ((3, 4), (32, 16, 32), (1, 1, 1), 64, (4, 8)),
+                                                           ((3, 4), (64, 16, 64), (2, 1, 2), 32, (4, 8))])
+def test_sweep_transpose_wave_kernel(ctx, layout, zones, zset, groups, quad, kernel):
+    """The seeded transpose identity with the wave-uniform plane kernel, and
+    with the systolic kernel keeping the faces between k-patches in LDS,
     installed (kr_problem_set_tuning): forward and adjoint sweeps both run it."""
     hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=groups, zones=zones, quad=quad, legendre=1))
     k = hp.kernels()
-    assert k.set_tuning(v_max=2) == 3
+    assert k.set_tuning(v_max=kernel[0]) == kernel[1]
     rng = np.random.default_rng(48)
     exits = [(s, PLANES[a]) for s in range(hp.num_sdoms) for a, e in enumerate(k.sdom_exits(s)) if e]
     x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)

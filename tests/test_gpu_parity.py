@@ -549,8 +549,9 @@ def test_nesting_invariance(ctx):
 
 
 # --------------------------------------------------------------------------
-# the wave-uniform plane kernel (an autotuner candidate for the nestings
-# whose fastest index is a group or a direction): installed explicitly
+# the autotuner's other sweep kernels for the nestings whose fastest index is
+# a group or a direction, installed explicitly: the wave-uniform plane kernel
+# and the systolic kernel with the faces between k-patches in LDS
 # --------------------------------------------------------------------------
 WAVE_LAYOUTS = ("DZG", "GZD", "ZDG", "ZGD")
 # pairs per subdomain a multiple of 64, rows of 8/16/32 zones; (16, 8, 32)
@@ -559,21 +560,39 @@ WAVE_SHAPES = [dict(zones=(16, 8, 32), zset=(2, 1, 1), groups=32, quad=(4, 4)),
                dict(zones=(32, 16, 12), zset=(2, 2, 2), groups=64, quad=(4, 2))]
 
 
+# the other sweep kernels the autotuner may install there: v_max 2 the wave
+# kernel (kind 3), v_max 3 the systolic kernel with the LDS k-face ring (kind 4)
+KERNELS = {"waves": (2, 3), "kring": (3, 4)}
+# 32 x 16 x 32 subdomains with 4 directions per set: every j-row and 16 pairs
+# (group-fastest) x 4 k-layers, or 4 pairs x 16 k-layers, per patch, so the
+# ring carries faces across 8 or 2 k-patches; one and 2 x 1 x 2 zone sets
+KRING_SHAPES = [dict(zones=(32, 16, 32), zset=(1, 1, 1), groups=64, quad=(4, 8)),
+                dict(zones=(64, 16, 64), zset=(2, 1, 2), groups=32, quad=(4, 8))]
+CASES = [("waves", WAVE_SHAPES[0]), ("waves", WAVE_SHAPES[1]), ("kring", KRING_SHAPES[0]),
+         ("kring", KRING_SHAPES[1])]
+
+
+def _install(hp, kernel):
+    v, kind = KERNELS[kernel]
+    got = hp.kernels().set_tuning(v_max=v)
+    assert got == kind, "kernel %s does not cover this shape (installed kind %d)" % (kernel, got)
+
+
 def _waves(hp):
-    kind = hp.kernels().set_tuning(v_max=2)
-    assert kind == 3, "the wave kernel does not cover this shape"
+    _install(hp, "waves")
 
 
 @pytest.mark.parametrize("layout", WAVE_LAYOUTS)
-@pytest.mark.parametrize("shape", range(len(WAVE_SHAPES)))
-def test_wave_kernel_pipeline(ctx, layout, shape):
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_wave_kernel_pipeline(ctx, layout, case):
     """Pipelined sweep with the fused population: psi bit for bit against the
     fma oracle, the population against the oracle's compensated sum, and the
     plane chunks on the domain's downwind boundary."""
-    kw = kwargs_for(layout, **WAVE_SHAPES[shape])
+    kernel, shape = CASES[case]
+    kw = kwargs_for(layout, **shape)
     hp, of, orf = _trio(ctx, **kw)
-    _waves(hp)
-    rng = np.random.default_rng(60 + shape)
+    _install(hp, kernel)
+    rng = np.random.default_rng(60 + case)
     seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
     pop = hp.kernels().sweep_solver_population(list(range(hp.num_sdoms)))
     of.sweep_solver()
@@ -588,14 +607,15 @@ def test_wave_kernel_pipeline(ctx, layout, shape):
                 assert np.array_equal(hp.get(name, t), of.get(name, t)), (t, name)
 
 
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
 @pytest.mark.parametrize("layout", WAVE_LAYOUTS)
-def test_wave_kernel_block_jacobi_and_plain(ctx, layout):
+def test_wave_kernel_block_jacobi_and_plain(ctx, layout, kernel):
     """Block Jacobi (every subdomain from zero incoming faces) and the plain
     pipelined sweep (no population) with the wave kernel, against the fma
     oracle bit for bit."""
-    kw = kwargs_for(layout, **WAVE_SHAPES[0])
+    kw = kwargs_for(layout, **(WAVE_SHAPES[0] if kernel == "waves" else KRING_SHAPES[0]))
     hp, of, orf = _trio(ctx, **kw)
-    _waves(hp)
+    _install(hp, kernel)
     rng = np.random.default_rng(64)
     seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
     for name in ("i_plane", "j_plane", "k_plane"):
@@ -610,14 +630,15 @@ def test_wave_kernel_block_jacobi_and_plain(ctx, layout):
     _assert_chunks(hp, of, None, "psi", hp.num_sdoms, exact=True)
 
 
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
 @pytest.mark.parametrize("layout", WAVE_LAYOUTS)
-def test_wave_kernel_iterations(ctx, layout):
+def test_wave_kernel_iterations(ctx, layout, kernel):
     """Three on-device source iterations with the wave kernel: phi and the
     populations equal those of the default (systolic) kernel bit for bit,
     and the populations agree with the reference-arithmetic oracle."""
-    kw = kwargs_for(layout, **WAVE_SHAPES[1])
+    kw = kwargs_for(layout, **(WAVE_SHAPES[1] if kernel == "waves" else KRING_SHAPES[1]))
     a, b = product_for(ctx, **kw), product_for(ctx, **kw)
-    _waves(b)
+    _install(b, kernel)
     orf = oracle_for("ref", **kw)
     for x in (a, b, orf):
         x.init_psi()
