@@ -288,17 +288,20 @@ int kr_diff_backward(kr_diff *diff, int32_t first, int32_t num, const double *co
  * this problem's own fields, installs the fastest, and caches the choice on
  * the context for every later problem of the same shape.  It runs the sweep,
  * so psi and the planes are overwritten: call it before the solve (or before
- * kr_batch_create).  choice (may be NULL) receives {kernel kind, zones per
+ * kr_batch_create).  The candidates are timed on the problem's current rhs:
+ * after one iteration (nonzero sources and fluxes) the ranking is the one
+ * the solve sees, while an all-zero rhs can rank them differently (the
+ * device draws less power moving zeros).  choice (may be NULL) receives {kernel kind, zones per
  * lane or patch columns, unit size in percent of the default}. */
 int kr_problem_autotune(kr_problem *prob, int32_t *choice);
 /* No counterpart in the reference.  Installs one point of the autotuner's
  * search space for the pipelined sweep instead of timing them all:
  * v_max caps the row-scan kernel's zones per lane (zone-fastest nestings,
  * 0 = no cap) or, in the other nestings, selects the kernel (0 = systolic
- * pencils, 2 = wave-uniform planes, 3 = systolic pencils with patches over
- * every j-row and the faces between k-patches in LDS, each where the shape
- * fits it); cap_div shrinks the systolic patch, the wave kernel's waves per
- * workgroup or kernel 3's pairs per zone column (16 / cap_div);
+ * pencils, 2 = wave-uniform planes, 3 = systolic pencils with the patches
+ * in k-fastest order and the faces between k-patches in LDS, each where the
+ * shape fits it); cap_div shrinks the systolic patch (both forms) or the
+ * wave kernel's waves per workgroup (16 / cap_div);
  * r_scale_pct scales the pair groups per work unit.  kr_problem_sweep_kind
  * reports the kernel installed: 0 systolic, 1 row scan, 3 wave-uniform,
  * 4 systolic with the LDS k-face ring. */

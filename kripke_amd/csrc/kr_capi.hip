@@ -1091,9 +1091,9 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
       } else {
         // the systolic kernel with its largest patch or half of it (v_max 0),
         // the wave kernel with 16 or 8 waves per workgroup (v_max 2), the
-        // systolic kernel with full-j patches and the LDS k-face ring, at most
-        // 16 or 8 pairs per zone column (v_max 3); shapes a kernel does not
-        // cover fall back to the plain systolic one
+        // systolic kernel with its patches in k-fastest order and the faces
+        // between k-patches in an LDS ring, 1024 or 512 threads (v_max 3);
+        // shapes a kernel does not cover fall back to the plain systolic one
         for (int v : {0, 2, 3})
           for (int cd : {1, 2}) {
             TuneChoice c;

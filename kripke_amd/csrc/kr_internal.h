@@ -129,7 +129,7 @@ struct SweepShape {
   int32_t inner_is_g;           // pair index = outer*inner + q ordering
   int32_t ref_mode;             // 1: read/write own planes only (Kernel::sweepSubdomain)
   int32_t nd, ni, nj, nk;       // subdomain extents
-  int32_t kring;                // systolic kernel with PJ = nj: faces between k-patches stay in an LDS ring
+  int32_t kring;                // systolic kernel, patches in k-fastest order: faces between k-patches in an LDS ring
 };
 
 void set_error(const std::string &msg);

@@ -107,8 +107,9 @@ int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count);
 // shrinks the systolic kernel's largest patch by that factor.
 struct SweepTune {
   int v_max = 0;           // row-scan: zones per lane cap; other nestings: 2 = the wave kernel, 3 = the systolic
-                           // kernel with full-j patches and an LDS k-face ring, where they fit
-  int cap_div = 1;         // systolic: patch size divisor; wave kernel: waves per workgroup = 16 / cap_div
+                           // kernel with k-fastest patches and an LDS k-face ring, where they fit
+  int cap_div = 1;         // systolic (both forms): patch size divisor; wave kernel: waves per workgroup
+                           // = 16 / cap_div
   size_t lds_max = 65536;  // the device's LDS per workgroup (shapes within 64 KiB are preferred)
 };
 // Wave-uniform plane sweep of the nestings whose fastest index is a group or

@@ -39,7 +39,8 @@ namespace kr {
 
 namespace {
 
-constexpr int COL_RING = 4;  // steps of prefetch
+constexpr int COL_RING = 2;  // steps of prefetch (each step more costs 10 VGPRs; at 4 steps, 98 VGPRs, the
+                             // 64^3-zone 128-group 128-direction sweep ran 1-2 ms slower per iteration)
 
 typedef __amdgpu_buffer_rsrc_t Rsrc;
 constexpr uint32_t NOWHERE = 0x80000000u;  // resources are smaller than 2 GiB
@@ -77,84 +78,208 @@ struct ColArgs {
   int32_t pg_begin, R, inner_is_g;
 };
 
-// Position of one thread in its unit: u = ((r*NPK + pk)*NPJ + pj)*ni + ii,
+// Position of one thread in its unit: u = ((r*NPK + pk)*NPJ + pj)*ni + ii
+// (k-ring variant: ((r*NPJ + pj)*NPK + pk)*ni + ii),
 // and the byte offsets of its current zone.  Before its first step a thread
 // rests on u = 0; past the last pair group it keeps cycling the last one, so
-// every offset stays in bounds.
+// every offset stays in bounds.  Offsets move by additions only: one step
+// along the pencil adds the per-step increments, the end of a pencil adds
+// one row of the transition table (next j-patch, next k-patch, or back to
+// the first patch), and a new pair group adds one of two pair rows (inner
+// index advanced, or wrapped with the outer index advanced).
 struct ColCursor {
   int32_t pos;               // steps since this thread's start (negative: not started)
   int32_t ii, pj, pk, r;
-  int32_t d, g, i, j, k;
+  int32_t gs;                // first inner index of the current pair group
   uint32_t psi8, sig8;       // 8*(d*fa + g*fg + z*fz), 8*(g*sg + z*sz)
   uint32_t x8, y8, z8;       // face offsets: i-plane row (k,j); j-plane (k,i); k-plane (j,i)
   uint32_t vol8;             // 8*z
+  int32_t ca, cb, cc, cw;    // coefficient table entries of a_i, b_j, c_k and w_d
 };
 
 struct ColConst {
   int Q, PJ, PK, NPJ, NPK;
   int q, jj, kk;
   int istep;                 // +1 / -1: zone i per step
+  int inner;                 // extent of the inner pair index (ng or nd)
   int32_t dpsi8, dsig8, dy8, dz8, dvol8;  // per-step offset increments
 };
 
-__device__ __forceinline__ void col_pair(ColCursor &c, const ColArgs &P, const ColConst &K) {
-  const int pidx = (P.pg_begin + min(c.r, P.R - 1)) * K.Q + K.q;
-  if (P.inner_is_g) {
-    c.d = pidx / P.ng;
-    c.g = pidx - c.d * P.ng;
+// Transition table (int32, in LDS): rows of COL_TAB_W entries.
+//   rows 0-2, end of a pencil: next patch along the fast patch axis (j, or k
+//     in k-fastest order) / next along the other axis / first patch again:
+//     psi8, sig8, x8, y8, z8, vol8 increments and the i, j, k index changes
+//   rows 3-4, new pair group: inner index + Q / wrapped, outer index + 1:
+//     psi8, sig8, x8, y8, z8 increments and the coefficient record change
+constexpr int COL_TAB_W = 9;
+constexpr int COL_TAB_INTS = 5 * COL_TAB_W;
+constexpr int COL_TAB_DOUBLES = 24;  // LDS reserved for the table (>= COL_TAB_INTS ints)
+static_assert(2 * COL_TAB_DOUBLES >= COL_TAB_INTS, "transition table");
+
+// KFAST: patches in k-fastest order (rows 0 and 1 move along k and j).
+template <bool KFAST>
+__device__ __forceinline__ void col_table(int32_t *tab, const ColArgs &P, const ColConst &K, int L) {
+  const int e = threadIdx.x;
+  if (e >= COL_TAB_INTS) return;
+  const int row = e / COL_TAB_W, col = e - row * COL_TAB_W;
+  const int jd = P.jd > 0 ? 1 : -1, kd = P.kd > 0 ? 1 : -1;
+  int32_t v = 0;
+  if (row < 3) {
+    const int Di = -(P.ni - 1) * K.istep;
+    const int jback = -jd * K.PJ * (K.NPJ - 1), kback = -kd * K.PK * (K.NPK - 1);
+    int Dj, Dk;
+    if (KFAST) {
+      Dk = row == 0 ? kd * K.PK : kback;
+      Dj = row == 0 ? 0 : (row == 1 ? jd * K.PJ : jback);
+    } else {
+      Dj = row == 0 ? jd * K.PJ : jback;
+      Dk = row == 0 ? 0 : (row == 1 ? kd * K.PK : kback);
+    }
+    const int Dz = (Dk * P.nj + Dj) * P.ni + Di;
+    switch (col) {
+      case 0: v = 8 * Dz * P.fz; break;
+      case 1: v = 8 * Dz * P.sz; break;
+      case 2: v = 8 * (Dk * P.nj + Dj) * P.pf[0]; break;
+      case 3: v = 8 * (Dk * P.ni + Di) * P.pf[1]; break;
+      case 4: v = 8 * (Dj * P.ni + Di) * P.pf[2]; break;
+      case 5: v = 8 * Dz; break;
+      case 6: v = Di; break;
+      case 7: v = Dj; break;
+      default: v = Dk; break;
+    }
   } else {
-    c.g = pidx / P.nd;
-    c.d = pidx - c.g * P.nd;
-  }
-}
-
-// Offsets of the first zone of the current patch row (ii = 0).
-__device__ __forceinline__ void col_patch(ColCursor &c, const ColArgs &P, const ColConst &K) {
-  const int jl = c.pj * K.PJ + K.jj, kl = c.pk * K.PK + K.kk;
-  c.j = P.jd > 0 ? jl : P.nj - 1 - jl;
-  c.k = P.kd > 0 ? kl : P.nk - 1 - kl;
-  c.i = K.istep > 0 ? 0 : P.ni - 1;
-  // relative offsets stay below 2 GiB (host-checked unit spans): 32-bit math
-  const uint32_t z = ((uint32_t)c.k * P.nj + c.j) * P.ni + c.i;
-  const int dd = c.d - P.d0, dg = c.g - P.g0;
-  c.psi8 = 8u * (uint32_t)(dd * P.fa + dg * P.fg + (int)z * P.fz);
-  c.sig8 = 8u * (uint32_t)(c.g * P.sg + (int)z * P.sz);
-  c.x8 = 8u * (uint32_t)(dd * P.pa[0] + dg * P.pg[0] + (c.k * P.nj + c.j) * P.pf[0]);
-  c.y8 = 8u * (uint32_t)(dd * P.pa[1] + dg * P.pg[1] + (c.k * P.ni + c.i) * P.pf[1]);
-  c.z8 = 8u * (uint32_t)(dd * P.pa[2] + dg * P.pg[2] + (c.j * P.ni + c.i) * P.pf[2]);
-  c.vol8 = 8u * z;
-}
-
-__device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const ColConst &K, int pos) {
-  c.pos = pos;
-  c.ii = c.pj = c.pk = c.r = 0;
-  col_pair(c, P, K);
-  col_patch(c, P, K);
-}
-
-__device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const ColConst &K) {
-  if (++c.pos <= 0) return;
-  if (++c.ii < P.ni) {
-    c.i += K.istep;
-    c.psi8 += K.dpsi8;
-    c.sig8 += K.dsig8;
-    c.y8 += K.dy8;
-    c.z8 += K.dz8;
-    c.vol8 += K.dvol8;
-    return;
-  }
-  c.ii = 0;
-  if (++c.pj == K.NPJ) {
-    c.pj = 0;
-    if (++c.pk == K.NPK) {
-      c.pk = 0;
-      if (c.r + 1 < P.R) {  // past the last pair group: stay on it
-        ++c.r;
-        col_pair(c, P, K);
-      }
+    int Ddd, Ddg;
+    if (P.inner_is_g) {
+      Ddd = row == 3 ? 0 : 1;
+      Ddg = row == 3 ? K.Q : K.Q - P.ng;
+    } else {
+      Ddd = row == 3 ? K.Q : K.Q - P.nd;
+      Ddg = row == 3 ? 0 : 1;
+    }
+    switch (col) {
+      case 0: v = 8 * (Ddd * P.fa + Ddg * P.fg); break;
+      case 1: v = 8 * Ddg * P.sg; break;
+      case 2: v = 8 * (Ddd * P.pa[0] + Ddg * P.pg[0]); break;
+      case 3: v = 8 * (Ddd * P.pa[1] + Ddg * P.pg[1]); break;
+      case 4: v = 8 * (Ddd * P.pa[2] + Ddg * P.pg[2]); break;
+      case 5: v = Ddd * L; break;
+      default: v = 0; break;
     }
   }
-  col_patch(c, P, K);
+  tab[e] = v;
+}
+
+// Cursor at unit position 0 (the unit's first pair group, first patch, first zone).
+__device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const ColConst &K, int pos, int L) {
+  c.pos = pos;
+  c.ii = c.pj = c.pk = c.r = 0;
+  const int pidx = P.pg_begin * K.Q + K.q;
+  int d, g;
+  if (P.inner_is_g) {
+    d = pidx / P.ng;
+    g = pidx - d * P.ng;
+    c.gs = g - K.q;
+  } else {
+    g = pidx / P.nd;
+    d = pidx - g * P.nd;
+    c.gs = d - K.q;
+  }
+  const int j = P.jd > 0 ? K.jj : P.nj - 1 - K.jj, k = P.kd > 0 ? K.kk : P.nk - 1 - K.kk;
+  const int i = K.istep > 0 ? 0 : P.ni - 1;
+  // relative offsets stay below 2 GiB (host-checked unit spans): 32-bit math
+  const uint32_t z = ((uint32_t)k * P.nj + j) * P.ni + i;
+  const int dd = d - P.d0, dg = g - P.g0;
+  c.psi8 = 8u * (uint32_t)(dd * P.fa + dg * P.fg + (int)z * P.fz);
+  c.sig8 = 8u * (uint32_t)(g * P.sg + (int)z * P.sz);
+  c.x8 = 8u * (uint32_t)(dd * P.pa[0] + dg * P.pg[0] + (k * P.nj + j) * P.pf[0]);
+  c.y8 = 8u * (uint32_t)(dd * P.pa[1] + dg * P.pg[1] + (k * P.ni + i) * P.pf[1]);
+  c.z8 = 8u * (uint32_t)(dd * P.pa[2] + dg * P.pg[2] + (j * P.ni + i) * P.pf[2]);
+  c.vol8 = 8u * z;
+  c.ca = d * L + i;
+  c.cb = d * L + P.ni + j;
+  c.cc = d * L + P.ni + P.nj + k;
+  c.cw = d * L + L - 1;
+}
+
+// COEF: also move the coefficient entries and the volume offset (the
+// compute cursor; the prefetch cursor needs neither).  KFAST: patches in
+// k-fastest order.  Every field is
+// written once, after the branches, so that each stays a register (stores
+// to different fields at the ends of different branches may otherwise be
+// merged into one store through a selected address, which puts the cursor
+// in scratch memory).
+template <bool COEF, bool KFAST>
+__device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const ColConst &K, const int32_t *tab) {
+  if (++c.pos <= 0) return;
+  int32_t dpsi = K.dpsi8, dsig = K.dsig8, dx = 0, dy = K.dy8, dz = K.dz8;
+  int32_t dvol = K.dvol8, da = K.istep, db = 0, dc = 0, dw = 0;
+  int32_t ii = c.ii + 1, pj = c.pj, pk = c.pk, r = c.r, gs = c.gs;
+  if (ii == P.ni) {  // end of a pencil
+    ii = 0;
+    int row = 0;
+    if (KFAST) {
+      if (++pk == K.NPK) {
+        pk = 0;
+        row = 1;
+        if (++pj == K.NPJ) {
+          pj = 0;
+          row = 2;
+        }
+      }
+    } else if (++pj == K.NPJ) {
+      pj = 0;
+      row = 1;
+      if (++pk == K.NPK) {
+        pk = 0;
+        row = 2;
+      }
+    }
+    const int32_t *t = tab + row * COL_TAB_W;
+    dpsi = t[0];
+    dsig = t[1];
+    dx = t[2];
+    dy = t[3];
+    dz = t[4];
+    dvol = t[5];
+    da = t[6];
+    db = t[7];
+    dc = t[8];
+    if (row == 2 && r + 1 < P.R) {  // past the last pair group: stay on it
+      ++r;
+      gs += K.Q;
+      const int32_t *u = tab + 3 * COL_TAB_W;
+      if (gs == K.inner) {
+        gs = 0;
+        u += COL_TAB_W;
+      }
+      dpsi += u[0];
+      dsig += u[1];
+      dx += u[2];
+      dy += u[3];
+      dz += u[4];
+      da += u[5];
+      db += u[5];
+      dc += u[5];
+      dw = u[5];
+    }
+  }
+  c.ii = ii;
+  c.pj = pj;
+  c.pk = pk;
+  c.r = r;
+  c.gs = gs;
+  c.psi8 += dpsi;
+  c.sig8 += dsig;
+  c.x8 += dx;
+  c.y8 += dy;
+  c.z8 += dz;
+  if (COEF) {
+    c.vol8 += dvol;
+    c.ca += da;
+    c.cb += db;
+    c.cc += dc;
+    c.cw += dw;
+  }
 }
 
 template <bool POP, bool ADJ, bool KRING>
@@ -164,13 +289,14 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   const int tid = threadIdx.x;
   const int ni = P.ni, nj = P.nj, nk = P.nk, nd = P.nd, ng = P.ng;
   const int L = ni + nj + nk + 1;  // coefficient record per direction: A[ni], B[nj], C[nk], w
-  double *fyb = lds;               // [2][nt]
-  double *fzb = lds + 2 * nt;      // [2][nt]
-  double *co = lds + 4 * nt;       // [nd][L]
-  // KRING (a patch spans every j-row): the faces between k-patches, [ni][PJ][Q]
-  // by the zone's step along its pencil; the top layer of patch pk writes a
-  // slot ni - PK + 1 steps before the bottom layer of patch pk + 1 reads it,
-  // and rewrites it only after that read
+  int32_t *tab = (int32_t *)lds;  // the cursor transition table
+  double *fyb = lds + COL_TAB_DOUBLES;  // [2][nt]
+  double *fzb = fyb + 2 * nt;           // [2][nt]
+  double *co = fyb + 4 * nt;            // [nd][L]
+  // KRING (patches in k-fastest order): the faces between k-patches,
+  // [ni][PJ][Q] by the zone's step along its pencil; the top layer of patch
+  // (pj, pk) writes a slot ni - PK + 1 steps before the bottom layer of the
+  // next patch, (pj, pk + 1), reads it, and rewrites it only after that read
   double *kring = co + nd * L;
   {
     const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], contiguous
@@ -190,6 +316,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     }
     if (POP)
       for (int d = tid; d < nd; d += nt) co[d * L + L - 1] = gp(P.w)[d];
+    col_table<KRING>(tab, P, K, L);
   }
   __syncthreads();
 
@@ -214,7 +341,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
 #pragma unroll
   for (int r = 0; r < RING; ++r) rx[r] = ry[r] = rz[r] = 0.0;
   ColCursor lc, cc;
-  col_init(lc, P, K, -(K.jj + K.kk));
+  col_init(lc, P, K, -(K.jj + K.kk), L);
   cc = lc;
 
   auto refill = [&](int r) {
@@ -223,10 +350,17 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     // incoming faces: x at a pencil start; y for the first j-row of a patch
     // (the subdomain's incoming face on the first patch, else the previous
     // patch's outgoing face in the own chunk); z likewise
-    if (lc.ii == 0) rx[r] = ld<0>(Rin0, lc.x8);
-    if (K.jj == 0) ry[r] = ld<0>(Rown1, (lc.pj > 0 || P.has_in[1]) ? lc.y8 : NOWHERE);
-    if (K.kk == 0) rz[r] = ld<0>(Rown2, ((!KRING && lc.pk > 0) || (lc.pk == 0 && P.has_in[2])) ? lc.z8 : NOWHERE);
-    col_step(lc, P, K);
+    // (each ring slot written on every path: conditional stores to different
+    // slots may be merged into one through a selected address, which puts
+    // the ring in scratch memory)
+    double vx = rx[r], vy = ry[r], vz = rz[r];
+    if (lc.ii == 0) vx = ld<0>(Rin0, lc.x8);
+    if (K.jj == 0) vy = ld<0>(Rown1, (lc.pj > 0 || P.has_in[1]) ? lc.y8 : NOWHERE);
+    if (K.kk == 0) vz = ld<0>(Rown2, ((!KRING && lc.pk > 0) || (lc.pk == 0 && P.has_in[2])) ? lc.z8 : NOWHERE);
+    rx[r] = vx;
+    ry[r] = vy;
+    rz[r] = vz;
+    col_step<false, KRING>(lc, P, K, tab);
   };
 
   // zone volumes: one value for the whole subdomain (the generator's meshes:
@@ -241,20 +375,27 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   const int rows = P.R * K.NPK * K.NPJ * ni;
   const int T = rows + (K.PJ - 1) + (K.PK - 1);
   double fx = 0.0, pacc = 0.0;
+  // LDS slots of the upwind neighbours' faces, read on every step and used
+  // where the thread has such a neighbour (a thread on the patch's first row
+  // reads its own slot and takes the prefetched face instead)
+  const int iy = K.jj == 0 ? tid : tid - K.Q, iz = K.kk == 0 ? tid : tid - K.Q * K.PJ;
 
   for (int t0 = 0; t0 < T; t0 += RING) {
 #pragma unroll
     for (int r = 0; r < RING; ++r) {
       const int cur = (t0 + r) & 1, prev = cur ^ 1;
       const bool act = cc.pos >= 0 && cc.pos < rows;
-      const double *rec = co + cc.d * L;
-      const double a = rec[cc.i];
-      const double bj = rec[ni + cc.j];
-      const double ck = rec[ni + nj + cc.k];
+      const double a = co[cc.ca];
+      const double bj = co[cc.cb];
+      const double ck = co[cc.cc];
       if (cc.ii == 0) fx = rx[r];
-      const double fy = K.jj == 0 ? ry[r] : fyb[prev * nt + tid - K.Q];
-      double fzv = K.kk == 0 ? rz[r] : fzb[prev * nt + tid - K.Q * K.PJ];
-      if (KRING && K.kk == 0 && cc.pk > 0) fzv = kring[(cc.ii * K.PJ + K.jj) * K.Q + K.q];
+      const double ly = fyb[prev * nt + iy], lz = fzb[prev * nt + iz];
+      const double fy = K.jj == 0 ? ry[r] : ly;
+      double fzv = K.kk == 0 ? rz[r] : lz;
+      if (KRING) {
+        const double lr = kring[(cc.ii * K.PJ + K.jj) * K.Q + K.q];
+        if (K.kk == 0 && cc.pk > 0) fzv = lr;
+      }
       // adjoint: incoming faces weigh 2, outgoing faces a, b, c (transposed zone relations)
       const double wx = ADJ ? 2.0 : a, wy = ADJ ? 2.0 : bj, wz = ADJ ? 2.0 : ck;
       const double num = __fma_rn(fzv, wz, __fma_rn(fy, wy, __fma_rn(fx, wx, rq[r])));
@@ -265,28 +406,34 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       const double fyo = (ADJ ? bj * p : p2) - fy, fzo = (ADJ ? ck * p : p2) - fzv;
       st<NT>(Rpsi, act ? cc.psi8 : NOWHERE, p);
       if (act && cc.ii == ni - 1) st<0>(Rout0, cc.x8, fx);
+      // both stores of a face issue with the unused one aimed NOWHERE: the
+      // lanes of a wave disagree on which buffer a face goes to, and a store
+      // through a per-lane choice of resource would loop over the choices
       if (act && K.jj == K.PJ - 1) {
-        if (cc.pj == K.NPJ - 1)
-          st<0>(Rout1, cc.y8, fyo);
-        else
-          st<0>(Rown1, cc.y8, fyo);
+        const bool last = cc.pj == K.NPJ - 1;
+        st<0>(Rout1, last ? cc.y8 : NOWHERE, fyo);
+        st<0>(Rown1, last ? NOWHERE : cc.y8, fyo);
       }
       if (act && K.kk == K.PK - 1) {
-        if (cc.pk == K.NPK - 1)
-          st<0>(Rout2, cc.z8, fzo);
-        else if (KRING)
-          kring[(cc.ii * K.PJ + K.jj) * K.Q + K.q] = fzo;
-        else
-          st<0>(Rown2, cc.z8, fzo);
+        const bool last = cc.pk == K.NPK - 1;
+        if (KRING) {
+          if (last)
+            st<0>(Rout2, cc.z8, fzo);
+          else
+            kring[(cc.ii * K.PJ + K.jj) * K.Q + K.q] = fzo;
+        } else {
+          st<0>(Rout2, last ? cc.z8 : NOWHERE, fzo);
+          st<0>(Rown2, last ? NOWHERE : cc.z8, fzo);
+        }
       }
       fyb[cur * nt + tid] = fyo;
       fzb[cur * nt + tid] = fzo;
       if (POP) {
-        const double t = (rec[L - 1] * p) * rv;
+        const double t = (co[cc.cw] * p) * rv;
         pacc += act ? t : 0.0;
         asm volatile("" : "+v"(pacc));  // summed in its step, not deferred
       }
-      col_step(cc, P, K);
+      col_step<true, KRING>(cc, P, K, tab);
       if (POP && !vol_uniform) rv = ld<0>(Rvol, cc.vol8);
       refill(r);
       __syncthreads();
@@ -368,6 +515,7 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   K.jj = (tid / sh.Q) % sh.PJ;
   K.kk = tid / (sh.Q * sh.PJ);
   K.istep = ((S.id > 0) != ADJ) ? 1 : -1;
+  K.inner = sh.inner_is_g ? S.ng : S.nd;
   K.dpsi8 = (int32_t)(8 * K.istep * S.fz);
   K.dsig8 = (int32_t)(8 * K.istep * S.sz);
   K.dy8 = (int32_t)(8 * K.istep * S.pf[1]);
@@ -484,27 +632,43 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     while (qv < 16 && inner % (qv * 2) == 0) qv *= 2;
     sh.Q = qv;
   }
-  // systolic kernel whose patches span every j-row (tuning point v_max = 3):
-  // the faces between k-patches stay in an LDS ring of [ni][nj][Q] doubles,
-  // so no face leaves the chip inside the subdomain; Q is capped at 16 /
-  // cap_div to fit the ring
+  // systolic kernel with its patches in k-fastest order (tuning point
+  // v_max = 3): the faces between k-patches stay in an LDS ring of
+  // [ni][PJ][Q] doubles, and only the faces between j-patches go through the
+  // own plane chunk (16/PJ bytes per zone and pair).  Among the shapes with
+  // the most columns within the thread budget (1024 / Q / cap_div), the
+  // widest PJ whose LDS fits: 64 KiB first (two workgroups per CU), then the
+  // device maximum.  Distances: a ring slot is written ni - PK + 1 steps
+  // before the next k-patch reads it (ni >= PK); a j-face leaves NPK*ni -
+  // PJ + 1 steps before the next j-patch uses it, and is read COL_RING steps
+  // ahead of that (NPK*ni >= PJ + COL_RING).
   if (!zfast && tune.v_max == 3) {
-    const int qk = std::max(1, std::min(sh.Q, 16 / std::max(1, tune.cap_div)));
     const size_t coef = (size_t)nd * (ni + nj + nk + 1);
-    for (int PK = largest_divisor_le(nk, 1024 / std::max(1, qk * nj)); PK >= 1 && qk * nj <= 1024;
-         PK = largest_divisor_le(nk, PK - 1)) {
-      const int nt = qk * nj * PK;
-      const size_t lds = (4 * (size_t)nt + coef + (size_t)ni * nj * qk) * sizeof(double);
-      if ((nk / PK == 1 || ni >= PK + COL_RING) && lds <= tune.lds_max) {
-        sh.Q = qk;
-        sh.PJ = nj;
-        sh.PK = PK;
-        sh.NPJ = 1;
-        sh.NPK = nk / PK;
-        sh.kring = nk / PK > 1 ? 1 : 0;
+    const int cap = 1024 / sh.Q / std::max(1, tune.cap_div);
+    for (int pass = 0; pass < 2; ++pass) {
+      const size_t lim = pass == 0 ? std::min<size_t>(65536, tune.lds_max) : tune.lds_max;
+      int best_pj = 0, best_pk = 0;
+      for (int pj = largest_divisor_le(nj, cap); pj >= 1; pj = largest_divisor_le(nj, pj - 1)) {
+        for (int pk = largest_divisor_le(nk, cap / pj); pk >= 1; pk = largest_divisor_le(nk, pk - 1)) {
+          const size_t nt = (size_t)sh.Q * pj * pk;
+          const size_t lds = (COL_TAB_DOUBLES + 4 * nt + coef + (size_t)ni * pj * sh.Q) * sizeof(double);
+          const bool ok = ni >= pk && (nj == pj || (int64_t)(nk / pk) * ni >= pj + COL_RING) && lds <= lim;
+          if (ok && (pj * pk > best_pj * best_pk || (pj * pk == best_pj * best_pk && pj > best_pj))) {
+            best_pj = pj;
+            best_pk = pk;
+          }
+          if (pk == 1) break;
+        }
+        if (pj == 1) break;
+      }
+      if (best_pj > 0) {
+        sh.PJ = best_pj;
+        sh.PK = best_pk;
+        sh.NPJ = nj / best_pj;
+        sh.NPK = nk / best_pk;
+        sh.kring = sh.NPK > 1 ? 1 : 0;
         return sh;
       }
-      if (PK == 1) break;
     }
   }
   // LDS: j/k face exchange [4][threads] and the coefficient records [nd][ni+nj+nk+1]
@@ -539,7 +703,7 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
       if (pj == 1) break;
     }
     if (best_pj == 0) continue;
-    const size_t lds = coef_lds + 4 * (size_t)sh.Q * best_pj * best_pk * sizeof(double);
+    const size_t lds = coef_lds + (COL_TAB_DOUBLES + 4 * (size_t)sh.Q * best_pj * best_pk) * sizeof(double);
     if (lds > lds_limit) continue;
     sh.PJ = best_pj;
     sh.PK = best_pk;
@@ -557,7 +721,7 @@ size_t sweep_lds_bytes(const SweepShape &shape, int64_t coef_doubles) {
   const size_t nt = (size_t)shape.Q * shape.PJ * shape.PK;
   if (shape.kind == 3) return std::max(2 * nt + (size_t)coef_doubles, nt) * sizeof(double);
   const size_t ring = shape.kring ? (size_t)shape.ni * shape.PJ * shape.Q : 0;
-  return (4 * nt + (size_t)coef_doubles + ring) * sizeof(double);
+  return (COL_TAB_DOUBLES + 4 * nt + (size_t)coef_doubles + ring) * sizeof(double);
 }
 
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,

@@ -563,9 +563,11 @@ WAVE_SHAPES = [dict(zones=(16, 8, 32), zset=(2, 1, 1), groups=32, quad=(4, 4)),
 # the other sweep kernels the autotuner may install there: v_max 2 the wave
 # kernel (kind 3), v_max 3 the systolic kernel with the LDS k-face ring (kind 4)
 KERNELS = {"waves": (2, 3), "kring": (3, 4)}
-# 32 x 16 x 32 subdomains with 4 directions per set: every j-row and 16 pairs
-# (group-fastest) x 4 k-layers, or 4 pairs x 16 k-layers, per patch, so the
-# ring carries faces across 8 or 2 k-patches; one and 2 x 1 x 2 zone sets
+# 32 x 16 x 32 subdomains with 4 directions per set: 4 j-rows x 16 k-layers
+# x 16 pairs (group-fastest; four j-patches, two k-patches each), or every
+# j-row x 16 k-layers x 4 pairs (direction-fastest; two k-patches), so the
+# ring carries faces between k-patches in both patch orders; one and
+# 2 x 1 x 2 zone sets
 KRING_SHAPES = [dict(zones=(32, 16, 32), zset=(1, 1, 1), groups=64, quad=(4, 8)),
                 dict(zones=(64, 16, 64), zset=(2, 1, 2), groups=32, quad=(4, 8))]
 CASES = [("waves", WAVE_SHAPES[0]), ("waves", WAVE_SHAPES[1]), ("kring", KRING_SHAPES[0]),
