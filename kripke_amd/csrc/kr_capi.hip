@@ -439,8 +439,9 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
   // --- host copies of small tables (quadrature, mesh, sigs) --------------
   const int nd = s0.num_directions, ng = s0.num_groups, ni = s0.ni, nj = s0.nj, nk = s0.nk;
   const int64_t nz = (int64_t)ni * nj * nk;
-  // 2*cos/delta tables, refreshed on device before each sweep
-  const int64_t coef_per = (int64_t)nd * (ni + nj + nk);
+  // 2*cos/delta tables, refreshed on device before each sweep, each followed
+  // by its uniform-rows flag (k_sweep_coef)
+  const int64_t coef_per = (int64_t)nd * (ni + nj + nk) + 1;
   std::vector<double> coef(coef_per * p->nsdom, 0.0);
   p->h_sd.resize(p->nsdom);
   int rc = KR_OK;
@@ -631,6 +632,13 @@ int kr_problem_create(kr_context *ctx, const kr_problem_desc *desc, kr_problem *
       (rc = upload(ctx, &p->d_planes_bj, pbj)) ||
       (rc = upload(ctx, &p->d_units_ref, uref)) ||
       (rc = upload(ctx, &p->d_units_pipe, upipe))) {
+    kr_problem_destroy(p);
+    return rc;
+  }
+  // the tables and their flags as the fields hold them now (every sweep
+  // entry point refreshes them before it sweeps; the autotuner's timing
+  // sweeps read these)
+  if ((rc = launch_sweep_coef(ctx->stream, p->d_sd, 0, p->nsdom))) {
     kr_problem_destroy(p);
     return rc;
   }

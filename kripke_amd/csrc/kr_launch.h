@@ -121,7 +121,8 @@ bool waves_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj,
 int launch_sweep_waves(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
                        int32_t num_units, const SweepShape &shape, double *pop, bool adjoint);
 // LDS bytes one sweep workgroup of `shape` takes, with coef_doubles of
-// coefficient records.
+// coefficient records (row-scan and wave kernels; the systolic kernel keeps
+// a fixed record per direction).
 size_t sweep_lds_bytes(const SweepShape &shape, int64_t coef_doubles);
 // aligned16: every psi/rhs/sigma_t/plane chunk starts on a 16-byte boundary.
 SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, int ref_mode,

@@ -90,37 +90,43 @@ struct ColArgs {
 struct ColCursor {
   int32_t pos;               // steps since this thread's start (negative: not started)
   int32_t ii, pj, pk, r;
-  int32_t gs;                // first inner index of the current pair group
   uint32_t psi8, sig8;       // 8*(d*fa + g*fg + z*fz), 8*(g*sg + z*sz)
   uint32_t x8, y8, z8;       // face offsets: i-plane row (k,j); j-plane (k,i); k-plane (j,i)
   uint32_t vol8;             // 8*z
-  int32_t ca, cb, cc, cw;    // coefficient table entries of a_i, b_j, c_k and w_d
+  int32_t ca, cb, cc, cw;    // LDS entries of a_i, b_j, c_k and w_d
 };
 
 struct ColConst {
   int Q, PJ, PK, NPJ, NPK;
   int q, jj, kk;
   int istep;                 // +1 / -1: zone i per step
-  int inner;                 // extent of the inner pair index (ng or nd)
+  int gpo, r0;               // pair groups per outer index (inner extent / Q); the unit's first group mod gpo
+  int rec;                   // LDS record per direction: COL_REC (uniform tables) or ni + nj + nk + 1
+  int cstep;                 // step of the a_i entry along a pencil: 0 (uniform tables) or istep
   int32_t dpsi8, dsig8, dy8, dz8, dvol8;  // per-step offset increments
 };
 
 // Transition table (int32, in LDS): rows of COL_TAB_W entries.
 //   rows 0-2, end of a pencil: next patch along the fast patch axis (j, or k
 //     in k-fastest order) / next along the other axis / first patch again:
-//     psi8, sig8, x8, y8, z8, vol8 increments and the i, j, k index changes
+//     psi8, sig8, x8, y8, z8, vol8 increments and the changes of the a_i,
+//     b_j, c_k entries in the LDS record (none with uniform tables)
 //   rows 3-4, new pair group: inner index + Q / wrapped, outer index + 1:
-//     psi8, sig8, x8, y8, z8 increments and the coefficient record change
+//     psi8, sig8, x8, y8, z8 increments and the change of LDS record
+//
+// LDS record per direction: with every row of the 2cos/delta tables holding
+// one value (the same zone width all along each axis, bit for bit: the
+// generator's meshes) the kernel keeps {a, b, c, w}; otherwise the whole
+// rows {A[ni], B[nj], C[nk], w} (kernel instance UNI = false).
 constexpr int COL_TAB_W = 9;
+constexpr int COL_REC = 4;
 constexpr int COL_TAB_INTS = 5 * COL_TAB_W;
 constexpr int COL_TAB_DOUBLES = 24;  // LDS reserved for the table (>= COL_TAB_INTS ints)
 static_assert(2 * COL_TAB_DOUBLES >= COL_TAB_INTS, "transition table");
 
 // KFAST: patches in k-fastest order (rows 0 and 1 move along k and j).
 template <bool KFAST>
-__device__ __forceinline__ void col_table(int32_t *tab, const ColArgs &P, const ColConst &K, int L) {
-  const int e = threadIdx.x;
-  if (e >= COL_TAB_INTS) return;
+__device__ __forceinline__ int32_t col_entry(int e, const ColArgs &P, const ColConst &K) {
   const int row = e / COL_TAB_W, col = e - row * COL_TAB_W;
   const int jd = P.jd > 0 ? 1 : -1, kd = P.kd > 0 ? 1 : -1;
   int32_t v = 0;
@@ -143,9 +149,9 @@ __device__ __forceinline__ void col_table(int32_t *tab, const ColArgs &P, const 
       case 3: v = 8 * (Dk * P.ni + Di) * P.pf[1]; break;
       case 4: v = 8 * (Dj * P.ni + Di) * P.pf[2]; break;
       case 5: v = 8 * Dz; break;
-      case 6: v = Di; break;
-      case 7: v = Dj; break;
-      default: v = Dk; break;
+      case 6: v = K.cstep != 0 ? Di : 0; break;
+      case 7: v = K.cstep != 0 ? Dj : 0; break;
+      default: v = K.cstep != 0 ? Dk : 0; break;
     }
   } else {
     int Ddd, Ddg;
@@ -162,15 +168,22 @@ __device__ __forceinline__ void col_table(int32_t *tab, const ColArgs &P, const 
       case 2: v = 8 * (Ddd * P.pa[0] + Ddg * P.pg[0]); break;
       case 3: v = 8 * (Ddd * P.pa[1] + Ddg * P.pg[1]); break;
       case 4: v = 8 * (Ddd * P.pa[2] + Ddg * P.pg[2]); break;
-      case 5: v = Ddd * L; break;
+      case 5: v = Ddd * K.rec; break;
       default: v = 0; break;
     }
   }
-  tab[e] = v;
+  return v;
+}
+
+// Every entry, whatever the workgroup size (a workgroup may have fewer
+// threads than the table has entries).
+template <bool KFAST>
+__device__ __forceinline__ void col_table(int32_t *tab, const ColArgs &P, const ColConst &K) {
+  for (int e = threadIdx.x; e < COL_TAB_INTS; e += blockDim.x) tab[e] = col_entry<KFAST>(e, P, K);
 }
 
 // Cursor at unit position 0 (the unit's first pair group, first patch, first zone).
-__device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const ColConst &K, int pos, int L) {
+__device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const ColConst &K, int pos) {
   c.pos = pos;
   c.ii = c.pj = c.pk = c.r = 0;
   const int pidx = P.pg_begin * K.Q + K.q;
@@ -178,11 +191,9 @@ __device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const C
   if (P.inner_is_g) {
     d = pidx / P.ng;
     g = pidx - d * P.ng;
-    c.gs = g - K.q;
   } else {
     g = pidx / P.nd;
     d = pidx - g * P.nd;
-    c.gs = d - K.q;
   }
   const int j = P.jd > 0 ? K.jj : P.nj - 1 - K.jj, k = P.kd > 0 ? K.kk : P.nk - 1 - K.kk;
   const int i = K.istep > 0 ? 0 : P.ni - 1;
@@ -195,10 +206,16 @@ __device__ __forceinline__ void col_init(ColCursor &c, const ColArgs &P, const C
   c.y8 = 8u * (uint32_t)(dd * P.pa[1] + dg * P.pg[1] + (k * P.ni + i) * P.pf[1]);
   c.z8 = 8u * (uint32_t)(dd * P.pa[2] + dg * P.pg[2] + (j * P.ni + i) * P.pf[2]);
   c.vol8 = 8u * z;
-  c.ca = d * L + i;
-  c.cb = d * L + P.ni + j;
-  c.cc = d * L + P.ni + P.nj + k;
-  c.cw = d * L + L - 1;
+  if (K.cstep != 0) {  // whole rows
+    c.ca = d * K.rec + i;
+    c.cb = d * K.rec + P.ni + j;
+    c.cc = d * K.rec + P.ni + P.nj + k;
+  } else {
+    c.ca = d * K.rec;
+    c.cb = d * K.rec + 1;
+    c.cc = d * K.rec + 2;
+  }
+  c.cw = d * K.rec + K.rec - 1;
 }
 
 // COEF: also move the coefficient entries and the volume offset (the
@@ -212,8 +229,8 @@ template <bool COEF, bool KFAST>
 __device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const ColConst &K, const int32_t *tab) {
   if (++c.pos <= 0) return;
   int32_t dpsi = K.dpsi8, dsig = K.dsig8, dx = 0, dy = K.dy8, dz = K.dz8;
-  int32_t dvol = K.dvol8, da = K.istep, db = 0, dc = 0, dw = 0;
-  int32_t ii = c.ii + 1, pj = c.pj, pk = c.pk, r = c.r, gs = c.gs;
+  int32_t dvol = K.dvol8, da = K.cstep, db = 0, dc = 0, dw = 0;
+  int32_t ii = c.ii + 1, pj = c.pj, pk = c.pk, r = c.r;
   if (ii == P.ni) {  // end of a pencil
     ii = 0;
     int row = 0;
@@ -246,12 +263,10 @@ __device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const C
     dc = t[8];
     if (row == 2 && r + 1 < P.R) {  // past the last pair group: stay on it
       ++r;
-      gs += K.Q;
-      const int32_t *u = tab + 3 * COL_TAB_W;
-      if (gs == K.inner) {
-        gs = 0;
-        u += COL_TAB_W;
-      }
+      // the new group starts an outer index when its number is a multiple
+      // of gpo (a remainder on the rare pair-group step instead of a
+      // register per lane)
+      const int32_t *u = tab + ((K.r0 + r) % K.gpo == 0 ? 4 : 3) * COL_TAB_W;
       dpsi += u[0];
       dsig += u[1];
       dx += u[2];
@@ -267,7 +282,6 @@ __device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const C
   c.pj = pj;
   c.pk = pk;
   c.r = r;
-  c.gs = gs;
   c.psi8 += dpsi;
   c.sig8 += dsig;
   c.x8 += dx;
@@ -282,13 +296,24 @@ __device__ __forceinline__ void col_step(ColCursor &c, const ColArgs &P, const C
   }
 }
 
-template <bool POP, bool ADJ, bool KRING>
+// UNI: the instance for uniform tables (and, with the population, one
+// volume for every zone); the other instance takes the whole table rows and
+// per-zone volumes.  Both are launched for every stage; each unit's
+// workgroup in the instance that does not match its subdomain returns at
+// once (the flag and the volume check are read from the device).
+template <bool POP, bool ADJ, bool KRING, bool UNI>
 __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, double *lds) {
   constexpr int RING = COL_RING;
+  const int ni = P.ni, nj = P.nj, nk = P.nk, nd = P.nd, ng = P.ng;
+  const int ncoef = nd * (ni + nj + nk);
+  const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], then the uniform flag
+  {
+    const bool fast = src[ncoef] != 0.0 && (!POP || P.vol_info[0] != 0.0);
+    if (fast != UNI) return;  // the whole workgroup: its subdomain is the other instance's
+  }
   const int nt = blockDim.x;
   const int tid = threadIdx.x;
-  const int ni = P.ni, nj = P.nj, nk = P.nk, nd = P.nd, ng = P.ng;
-  const int L = ni + nj + nk + 1;  // coefficient record per direction: A[ni], B[nj], C[nk], w
+  const int L = K.rec;
   int32_t *tab = (int32_t *)lds;  // the cursor transition table
   double *fyb = lds + COL_TAB_DOUBLES;  // [2][nt]
   double *fzb = fyb + 2 * nt;           // [2][nt]
@@ -298,9 +323,14 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   // (pj, pk) writes a slot ni - PK + 1 steps before the bottom layer of the
   // next patch, (pj, pk + 1), reads it, and rewrites it only after that read
   double *kring = co + nd * L;
-  {
-    const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], contiguous
-    for (int e = tid; e < nd * (ni + nj + nk); e += nt) {
+  if (UNI) {
+    for (int d = tid; d < nd; d += nt) {
+      co[d * L + 0] = src[d * ni];
+      co[d * L + 1] = src[nd * ni + d * nj];
+      co[d * L + 2] = src[nd * (ni + nj) + d * nk];
+    }
+  } else {
+    for (int e = tid; e < ncoef; e += nt) {
       int dst;
       if (e < nd * ni) {
         const int d = e / ni;
@@ -314,10 +344,9 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       }
       co[dst] = src[e];
     }
-    if (POP)
-      for (int d = tid; d < nd; d += nt) co[d * L + L - 1] = gp(P.w)[d];
-    col_table<KRING>(tab, P, K, L);
   }
+  for (int d = tid; d < nd; d += nt) co[d * L + L - 1] = POP ? gp(P.w)[d] : 0.0;
+  col_table<KRING>(tab, P, K);
   __syncthreads();
 
   // resources start at the unit's first pair; the host keeps each unit's
@@ -329,11 +358,17 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   const Rsrc Rrhs = rsrc(P.rhs + P.base_f, span(items * nz - P.base_f));
   const Rsrc Rpsi = rsrc(P.psi + P.base_f, span(items * nz - P.base_f));
   const Rsrc Rsig = rsrc(P.sig, span((int64_t)ng * nz));
-  const Rsrc Rvol = rsrc(P.vol, POP ? nz * 8 : 0u);
-  const Rsrc Rin0 = rsrc(P.own[0] + P.base_p[0], P.has_in[0] ? span(items * nf[0] - P.base_p[0]) : 0u);
+  // the tables and the zone volumes take plain global loads (always in
+  // range, two scalar registers of base each instead of a four-register
+  // resource: the kernel is short of scalar registers)
+  const gmem<double> *gvol = gp(P.vol);
+  // x-faces (one load at a pencil's start, one store at its end, both under
+  // a branch): plain global accesses from scalar bases
+  const gmem<char> *gin0 = (const gmem<char> *)gp(P.own[0] + P.base_p[0]);
+  gmem<char> *gout0 = (gmem<char> *)gp(P.out[0] + P.base_p[0]);
+  const bool in0 = P.has_in[0] != 0;
   const Rsrc Rown1 = rsrc(P.own[1] + P.base_p[1], span(items * nf[1] - P.base_p[1]));
   const Rsrc Rown2 = rsrc(P.own[2] + P.base_p[2], span(items * nf[2] - P.base_p[2]));
-  const Rsrc Rout0 = rsrc(P.out[0] + P.base_p[0], span(items * nf[0] - P.base_p[0]));
   const Rsrc Rout1 = rsrc(P.out[1] + P.base_p[1], span(items * nf[1] - P.base_p[1]));
   const Rsrc Rout2 = rsrc(P.out[2] + P.base_p[2], span(items * nf[2] - P.base_p[2]));
 
@@ -341,7 +376,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
 #pragma unroll
   for (int r = 0; r < RING; ++r) rx[r] = ry[r] = rz[r] = 0.0;
   ColCursor lc, cc;
-  col_init(lc, P, K, -(K.jj + K.kk), L);
+  col_init(lc, P, K, -(K.jj + K.kk));
   cc = lc;
 
   auto refill = [&](int r) {
@@ -354,7 +389,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     // slots may be merged into one through a selected address, which puts
     // the ring in scratch memory)
     double vx = rx[r], vy = ry[r], vz = rz[r];
-    if (lc.ii == 0) vx = ld<0>(Rin0, lc.x8);
+    if (lc.ii == 0) vx = in0 ? *(const gmem<double> *)(gin0 + lc.x8) : 0.0;
     if (K.jj == 0) vy = ld<0>(Rown1, (lc.pj > 0 || P.has_in[1]) ? lc.y8 : NOWHERE);
     if (K.kk == 0) vz = ld<0>(Rown2, ((!KRING && lc.pk > 0) || (lc.pk == 0 && P.has_in[2])) ? lc.z8 : NOWHERE);
     rx[r] = vx;
@@ -363,12 +398,12 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
     col_step<false, KRING>(lc, P, K, tab);
   };
 
-  // zone volumes: one value for the whole subdomain (the generator's meshes:
-  // no per-zone load), else one load per zone step
-  const bool vol_uniform = POP && P.vol_info[0] != 0.0;
+  // zone volumes: one value for the whole subdomain (the UNI instance), else
+  // one load per zone step, issued a step ahead and before that step's ring
+  // loads (so its wait does not wait for them)
   const double vol0 = POP ? P.vol_info[1] : 0.0;
   double rv = vol0;
-  if (POP && !vol_uniform) rv = ld<0>(Rvol, cc.vol8);
+  if (POP && !UNI) rv = gvol[cc.vol8 / 8u];
 #pragma unroll
   for (int r = 0; r < RING; ++r) refill(r);
 
@@ -405,7 +440,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
       fx = (ADJ ? a * p : p2) - fx;
       const double fyo = (ADJ ? bj * p : p2) - fy, fzo = (ADJ ? ck * p : p2) - fzv;
       st<NT>(Rpsi, act ? cc.psi8 : NOWHERE, p);
-      if (act && cc.ii == ni - 1) st<0>(Rout0, cc.x8, fx);
+      if (act && cc.ii == ni - 1) *(gmem<double> *)(gout0 + cc.x8) = fx;
       // both stores of a face issue with the unused one aimed NOWHERE: the
       // lanes of a wave disagree on which buffer a face goes to, and a store
       // through a per-lane choice of resource would loop over the choices
@@ -434,7 +469,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
         asm volatile("" : "+v"(pacc));  // summed in its step, not deferred
       }
       col_step<true, KRING>(cc, P, K, tab);
-      if (POP && !vol_uniform) rv = ld<0>(Rvol, cc.vol8);
+      if (POP && !UNI) rv = gvol[cc.vol8 / 8u];
       refill(r);
       __syncthreads();
     }
@@ -455,10 +490,10 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
 }  // namespace
 
 // ADJ: the transposed sweep (reads psi, writes rhs, every axis reversed).
-template <bool POP, bool ADJ, bool KRING>
-__global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes,
-                                                const SweepUnit *__restrict__ units, SweepShape sh,
-                                                double *__restrict__ pop) {
+template <bool POP, bool ADJ, bool KRING, bool UNI>
+__device__ __forceinline__ void sweep_unit(const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes,
+                                           const SweepUnit *__restrict__ units, const SweepShape &sh,
+                                           double *__restrict__ pop) {
   extern __shared__ double lds[];
   const SweepUnit U = units[blockIdx.x];
   const DevSdom &S = sd[U.sdom];
@@ -515,13 +550,33 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
   K.jj = (tid / sh.Q) % sh.PJ;
   K.kk = tid / (sh.Q * sh.PJ);
   K.istep = ((S.id > 0) != ADJ) ? 1 : -1;
-  K.inner = sh.inner_is_g ? S.ng : S.nd;
+  K.gpo = (sh.inner_is_g ? S.ng : S.nd) / sh.Q;
+  K.r0 = U.pg_begin % K.gpo;
   K.dpsi8 = (int32_t)(8 * K.istep * S.fz);
   K.dsig8 = (int32_t)(8 * K.istep * S.sz);
   K.dy8 = (int32_t)(8 * K.istep * S.pf[1]);
   K.dz8 = (int32_t)(8 * K.istep * S.pf[2]);
   K.dvol8 = 8 * K.istep;
-  cols_body<POP, ADJ, KRING>(P, K, lds);
+  K.rec = UNI ? COL_REC : S.ni + S.nj + S.nk + 1;
+  K.cstep = UNI ? 0 : K.istep;
+  cols_body<POP, ADJ, KRING, UNI>(P, K, lds);
+}
+
+// Held to 80 VGPRs (six waves per SIMD), so that three 512-thread
+// workgroups share a CU.
+template <bool ADJ, bool KRING, bool UNI>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void k_sweep(
+    const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes, const SweepUnit *__restrict__ units,
+    SweepShape sh, double *__restrict__ pop) {
+  sweep_unit<false, ADJ, KRING, UNI>(sd, planes, units, sh, pop);
+}
+
+// The solve's sweep, with the fused population.
+template <bool KRING, bool UNI>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_pop(
+    const DevSdom *__restrict__ sd, const SweepPlanes *__restrict__ planes, const SweepUnit *__restrict__ units,
+    SweepShape sh, double *__restrict__ pop) {
+  sweep_unit<true, false, KRING, UNI>(sd, planes, units, sh, pop);
 }
 
 // 2*cos(d)/delta per (direction, zone index), the exact expression of
@@ -529,10 +584,11 @@ __global__ __launch_bounds__(1024) void k_sweep(const DevSdom *__restrict__ sd, 
 // subdomains [first, first+count).  Recomputed before every sweep so that
 // the quadrature and mesh fields are read as they are at call time.
 __global__ __launch_bounds__(256) void k_sweep_coef(const DevSdom *__restrict__ sd, int first) {
-  const DevSdom &S = sd[first + blockIdx.y];
+  const DevSdom &S = sd[first + blockIdx.x];
   const int n[3] = {S.ni, S.nj, S.nk};
   const int total = S.nd * (n[0] + n[1] + n[2]);
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+  int same = 1;  // every row holds one value (bit for bit)
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int d = e / (n[0] + n[1] + n[2]);
     int r = e - d * (n[0] + n[1] + n[2]);
     int ax = 0;
@@ -540,8 +596,14 @@ __global__ __launch_bounds__(256) void k_sweep_coef(const DevSdom *__restrict__ 
       r -= n[ax];
       ++ax;
     }
-    gp(S.coef[ax])[d * n[ax] + r] = 2.0 * gp(S.cosines[ax])[d] / gp(S.delta[ax])[r];
+    const double c2 = 2.0 * gp(S.cosines[ax])[d];
+    const double v = c2 / gp(S.delta[ax])[r];
+    gp(S.coef[ax])[d * n[ax] + r] = v;
+    same &= __double_as_longlong(v) == __double_as_longlong(c2 / gp(S.delta[ax])[0]);
   }
+  same = __syncthreads_and(same);
+  // the flag after the tables: which instance of the systolic sweep takes the subdomain
+  if (threadIdx.x == 0) gp(S.coef[0])[total] = same ? 1.0 : 0.0;
 }
 
 int allow_lds(const void *kernel, size_t bytes) {
@@ -551,7 +613,7 @@ int allow_lds(const void *kernel, size_t bytes) {
 
 int launch_sweep_coef(hipStream_t st, const DevSdom *sd, int first, int count) {
   if (count <= 0) return KR_OK;
-  hipLaunchKernelGGL(k_sweep_coef, dim3(4, count), dim3(256), 0, st, sd, first);
+  hipLaunchKernelGGL(k_sweep_coef, dim3(count), dim3(256), 0, st, sd, first);
   KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }
@@ -643,7 +705,9 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
   // PJ + 1 steps before the next j-patch uses it, and is read COL_RING steps
   // ahead of that (NPK*ni >= PJ + COL_RING).
   if (!zfast && tune.v_max == 3) {
-    const size_t coef = (size_t)nd * (ni + nj + nk + 1);
+    const size_t coef = (size_t)nd * COL_REC;
+    // the instance for varying tables holds whole table rows: it must fit the device too
+    const size_t rows_extra = (size_t)nd * (ni + nj + nk + 1 - COL_REC) * sizeof(double);
     const int cap = 1024 / sh.Q / std::max(1, tune.cap_div);
     for (int pass = 0; pass < 2; ++pass) {
       const size_t lim = pass == 0 ? std::min<size_t>(65536, tune.lds_max) : tune.lds_max;
@@ -652,7 +716,8 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
         for (int pk = largest_divisor_le(nk, cap / pj); pk >= 1; pk = largest_divisor_le(nk, pk - 1)) {
           const size_t nt = (size_t)sh.Q * pj * pk;
           const size_t lds = (COL_TAB_DOUBLES + 4 * nt + coef + (size_t)ni * pj * sh.Q) * sizeof(double);
-          const bool ok = ni >= pk && (nj == pj || (int64_t)(nk / pk) * ni >= pj + COL_RING) && lds <= lim;
+          const bool ok = ni >= pk && (nj == pj || (int64_t)(nk / pk) * ni >= pj + COL_RING) && lds <= lim &&
+                          lds + rows_extra <= tune.lds_max;
           if (ok && (pj * pk > best_pj * best_pk || (pj * pk == best_pj * best_pk && pj > best_pj))) {
             best_pj = pj;
             best_pk = pk;
@@ -671,8 +736,9 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
       }
     }
   }
-  // LDS: j/k face exchange [4][threads] and the coefficient records [nd][ni+nj+nk+1]
-  const size_t coef_lds = (size_t)nd * (ni + nj + nk + 1) * sizeof(double);
+  // LDS: j/k face exchange [4][threads] and the direction records [nd][COL_REC]
+  const size_t coef_lds = (size_t)nd * COL_REC * sizeof(double);
+  const size_t rows_extra = (size_t)nd * (ni + nj + nk + 1 - COL_REC) * sizeof(double);
   for (int pass = 0; pass < 2; ++pass) {
   const size_t lds_limit = pass == 0 ? std::min<size_t>(65536, tune.lds_max) : tune.lds_max;
   for (int cap = 1024 / sh.Q / std::max(1, tune.cap_div); cap >= 1; cap /= 2) {
@@ -704,7 +770,7 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     }
     if (best_pj == 0) continue;
     const size_t lds = coef_lds + (COL_TAB_DOUBLES + 4 * (size_t)sh.Q * best_pj * best_pk) * sizeof(double);
-    if (lds > lds_limit) continue;
+    if (lds > lds_limit || lds + rows_extra > tune.lds_max) continue;
     sh.PJ = best_pj;
     sh.PK = best_pk;
     sh.NPJ = nj / best_pj;
@@ -712,16 +778,24 @@ SweepShape choose_sweep_shape(int nesting, int32_t nd, int32_t ng, int32_t ni, i
     return sh;
   }
   }
-  sh.kind = 2;  // no shape fits: directions x zone extents too large for the LDS coefficient table
+  sh.kind = 2;  // no shape fits the device's LDS
   return sh;
+}
+
+// LDS of one systolic workgroup: uniform tables (one record per direction)
+// or whole table rows.
+static size_t cols_lds_bytes(const SweepShape &shape, bool uni) {
+  const size_t nt = (size_t)shape.Q * shape.PJ * shape.PK;
+  const size_t ring = shape.kring ? (size_t)shape.ni * shape.PJ * shape.Q : 0;
+  const size_t rec = uni ? COL_REC : (size_t)shape.ni + shape.nj + shape.nk + 1;
+  return (COL_TAB_DOUBLES + 4 * nt + (size_t)shape.nd * rec + ring) * sizeof(double);
 }
 
 size_t sweep_lds_bytes(const SweepShape &shape, int64_t coef_doubles) {
   if (shape.kind == 1) return (2 * (size_t)shape.RG * shape.PK * shape.V + (size_t)coef_doubles) * sizeof(double);
   const size_t nt = (size_t)shape.Q * shape.PJ * shape.PK;
   if (shape.kind == 3) return std::max(2 * nt + (size_t)coef_doubles, nt) * sizeof(double);
-  const size_t ring = shape.kring ? (size_t)shape.ni * shape.PJ * shape.Q : 0;
-  return (COL_TAB_DOUBLES + 4 * nt + (size_t)coef_doubles + ring) * sizeof(double);
+  return cols_lds_bytes(shape, false);  // the larger of the systolic kernel's two instances
 }
 
 int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, const SweepUnit *units,
@@ -736,32 +810,37 @@ int launch_sweep(hipStream_t st, const DevSdom *sd, const SweepPlanes *planes, c
     return launch_sweep_rows(st, sd, planes, units, num_units, shape,
                              (int64_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1), pop, adjoint);
   const int nt = shape.Q * shape.PJ * shape.PK;
-  const size_t lds = sweep_lds_bytes(shape, (int64_t)shape.nd * (shape.ni + shape.nj + shape.nk + 1));
-#define KR_SWEEP_LAUNCH(KR)                                                                                        \
+  // both instances: each unit's workgroup in the one that does not match
+  // its subdomain's tables (uniform or not) returns at once
+#define KR_SWEEP_LAUNCH(KR, UNI)                                                                                   \
   do {                                                                                                             \
+    const size_t lds = cols_lds_bytes(shape, UNI);                                                                 \
     if (lds > 65536) {                                                                                             \
       int rc;                                                                                                      \
-      if ((rc = allow_lds((const void *)k_sweep<false, true, KR>, lds)) ||                                         \
-          (rc = allow_lds((const void *)k_sweep<true, false, KR>, lds)) ||                                         \
-          (rc = allow_lds((const void *)k_sweep<false, false, KR>, lds)))                                          \
+      if ((rc = allow_lds((const void *)k_sweep<true, KR, UNI>, lds)) ||                                           \
+          (rc = allow_lds((const void *)k_sweep_pop<KR, UNI>, lds)) ||                                             \
+          (rc = allow_lds((const void *)k_sweep<false, KR, UNI>, lds)))                                            \
         return rc;                                                                                                 \
     }                                                                                                              \
     if (adjoint)                                                                                                   \
-      hipLaunchKernelGGL((k_sweep<false, true, KR>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, \
+      hipLaunchKernelGGL((k_sweep<true, KR, UNI>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape,   \
                          pop);                                                                                     \
     else if (pop)                                                                                                  \
-      hipLaunchKernelGGL((k_sweep<true, false, KR>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape, \
+      hipLaunchKernelGGL((k_sweep_pop<KR, UNI>), dim3(num_units), dim3(nt), lds, st, sd, planes, units, shape,     \
                          pop);                                                                                     \
     else                                                                                                           \
-      hipLaunchKernelGGL((k_sweep<false, false, KR>), dim3(num_units), dim3(nt), lds, st, sd, planes, units,      \
+      hipLaunchKernelGGL((k_sweep<false, KR, UNI>), dim3(num_units), dim3(nt), lds, st, sd, planes, units,        \
                          shape, pop);                                                                              \
+    KR_HIP_CHECK(hipGetLastError());                                                                               \
   } while (0)
-  if (shape.kring)
-    KR_SWEEP_LAUNCH(true);
-  else
-    KR_SWEEP_LAUNCH(false);
+  if (shape.kring) {
+    KR_SWEEP_LAUNCH(true, true);
+    KR_SWEEP_LAUNCH(true, false);
+  } else {
+    KR_SWEEP_LAUNCH(false, true);
+    KR_SWEEP_LAUNCH(false, false);
+  }
 #undef KR_SWEEP_LAUNCH
-  KR_HIP_CHECK(hipGetLastError());
   return KR_OK;
 }
 

@@ -649,6 +649,10 @@ extern "C" int64_t ko_set(ko_problem *p, const char *name, int32_t chunk, const 
 #undef X
   if (n == "sigs") return copy_in(p->sigs, src);
   if (n == "volume" && chunk >= 0 && chunk < p->nr) return copy_in(p->rd[chunk].volume, src);
+  /* mesh spacing along each axis (whole mesh), read by the sweep per zone */
+  if (n == "dx") return copy_in(p->dx, src);
+  if (n == "dy") return copy_in(p->dy, src);
+  if (n == "dz") return copy_in(p->dz, src);
   return -1;
 }
 

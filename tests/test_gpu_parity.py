@@ -651,3 +651,50 @@ def test_wave_kernel_iterations(ctx, layout, kernel):
         assert abs(pa - pb) <= 1e-14 * abs(pa)
         for c in range(a.num_pr):
             assert np.array_equal(a.get("phi", c), b.get("phi", c)), (it, c)
+
+
+def _vary_mesh(hp, oracles, zset, rng):
+    """Zone widths that vary along every axis (the product holds one chunk
+    per zone set along the axis, consecutive pieces of the oracle's global
+    array) and zone volumes that vary per zone, written into the product and
+    the oracles alike."""
+    for ax, name in enumerate(("dx", "dy", "dz")):
+        full = oracles[0].get(name)
+        full = full * (0.75 + 0.5 * rng.random(full.size))
+        for o in oracles:
+            o.set(name, 0, full)
+        n = full.size // zset[ax]
+        for c in range(zset[ax]):
+            hp.set(name, c, full[c * n:(c + 1) * n])
+    for r in range(hp.num_r):
+        v = hp.get("volume", r)
+        v = v * (0.5 + rng.random(v.size))
+        hp.set("volume", r, v)
+        for o in oracles:
+            o.set("volume", r, v)
+
+
+@pytest.mark.parametrize("kernel", ["default", "kring"])
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_sweep_nonuniform_mesh(ctx, layout, kernel):
+    """Pipelined sweep with the fused population on a mesh whose zone widths
+    and volumes vary (the systolic kernel then reads its 2cos/delta tables
+    per zone instead of one record per direction, and each zone's volume):
+    psi bit for bit against the fma oracle (normwise for the row scan), the
+    population against the oracle's compensated sum."""
+    if kernel == "kring" and layout in ZFAST:
+        pytest.skip("the k-ring variant serves the other four nestings")
+    shape = KRING_SHAPES[0] if kernel == "kring" else dict(zones=(16, 8, 12), zset=(2, 2, 1), groups=32, quad=(4, 4))
+    kw = kwargs_for(layout, **shape)
+    hp, of, orf = _trio(ctx, **kw)
+    if kernel == "kring":
+        _install(hp, kernel)
+    rng = np.random.default_rng(70)
+    _vary_mesh(hp, [of, orf], shape["zset"], rng)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    pop = hp.kernels().sweep_solver_population(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
+    _, comp = orf.population()
+    assert abs(pop - comp) <= TOL_REF * abs(comp)
