@@ -101,7 +101,7 @@ def run_gpu(layout, steps, warmup):
     return elapsed, setup_s, timers, nb, tune
 
 
-SWEEP_KERNELS = {0: "k_sweep", 1: "k_sweep_rows", 3: "k_sweep_waves"}
+SWEEP_KERNELS = {0: "k_sweep_pop", 1: "k_sweep_rows", 3: "k_sweep_waves", 4: "k_sweep_pop (k-ring)"}
 
 
 def nesting_figures(layout, steps, elapsed, timers, nb, tune):

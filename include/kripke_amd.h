@@ -291,8 +291,9 @@ int kr_diff_backward(kr_diff *diff, int32_t first, int32_t num, const double *co
  * kr_batch_create).  The candidates are timed on the problem's current rhs:
  * after one iteration (nonzero sources and fluxes) the ranking is the one
  * the solve sees, while an all-zero rhs can rank them differently (the
- * device draws less power moving zeros).  choice (may be NULL) receives {kernel kind, zones per
- * lane or patch columns, unit size in percent of the default}. */
+ * device draws less power moving zeros).  choice (may be NULL) receives {kernel kind (as
+ * kr_problem_sweep_kind reports it), zones per lane or patch columns, unit size in percent of the
+ * default}. */
 int kr_problem_autotune(kr_problem *prob, int32_t *choice);
 /* No counterpart in the reference.  Installs one point of the autotuner's
  * search space for the pipelined sweep instead of timing them all:

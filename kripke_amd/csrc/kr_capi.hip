@@ -1164,7 +1164,7 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
   int rc = install_choice(p, best);
   if (rc) return rc;
   if (choice) {
-    choice[0] = p->shape_pipe.kind;
+    choice[0] = (p->shape_pipe.kind == 0 && p->shape_pipe.kring) ? 4 : p->shape_pipe.kind;  // as kr_problem_sweep_kind
     choice[1] = p->shape_pipe.kind == 1 ? p->shape_pipe.V : p->shape_pipe.PJ * p->shape_pipe.PK;
     choice[2] = best.r_scale_pct;
   }
