@@ -16,14 +16,22 @@
 // subdomain go to the downwind subdomain's plane chunk (pipeline mode) or
 // stay in the own chunk (Kernel::sweepSubdomain mode).
 //
-// Memory pipeline: every stream is a buffer resource; rhs, sigma_t and the
-// three incoming faces of a step are read RING steps ahead into a register
-// ring, unconditionally (a thread that needs no face reads NOWHERE, which
-// returns zero without a memory access), so the wait before a step's use
-// never drains the loads of later steps.  The 2cos/delta tables and the
-// quadrature weights sit in LDS.  A patch-boundary face is written at least
-// RING + 1 steps (and so at least one barrier) before the next patch's
-// prefetch reads it; the shape chooser enforces that distance.
+// Memory pipeline: rhs, sigma_t and the incoming faces of a step are read
+// RING steps ahead into a register ring (buffer resources; a thread that
+// needs no face reads NOWHERE, which returns zero without a memory access),
+// and the step loop issues no other global load, so the wait before a
+// step's use never drains the loads of later steps.  A patch-boundary face
+// is written at least RING + 1 steps (and so at least one barrier) before
+// the next patch's prefetch reads it; the shape chooser enforces that
+// distance.
+//
+// Tables: the 2cos/delta rows and the quadrature weights sit in LDS, one
+// record per direction: {a, b, c, w} when every row is uniform (the common
+// case), whole rows otherwise, in two kernel instances that are both
+// launched (each returns for the other's subdomains).
+//
+// k-ring variant (KRING): patches in k-fastest order, the faces between
+// k-patches kept in an LDS ring instead of the own plane chunk.
 //
 // Arithmetic per zone (bit-identical across nestings and schedules):
 //   num = fma(fz, c_k, fma(fy, b_j, fma(fx, a_i, rhs)))
