@@ -226,3 +226,47 @@ def test_batch_refuses_after_member_destroyed(ctx):
     # the surviving problem still runs on its own
     assert np.isfinite(hps[0].iterate(1))
     hps[0].close()
+
+
+def test_batch_systolic_tuning_points(ctx):
+    """Problems of the group- and direction-fastest nestings with the k-ring
+    variant (patches in k-fastest order, faces between k-patches in LDS) and
+    the wave-uniform kernel installed, two of them with varying zone widths
+    (the systolic kernel's instance for whole table rows) in one batch:
+    populations and phi equal each problem's own runs bit for bit."""
+    base = dict(zones=(32, 16, 32), zset=(1, 1, 1), groups=64, quad=(4, 8), legendre=2, dset=8, gset=1, geometry=1)
+    cfgs = [dict(base, layout=lay, seed=700 + i) for i, lay in enumerate(["DZG", "GZD", "ZDG", "ZGD", "DZG", "GZD"])]
+    tuning = [3, 3, 3, 3, 2, 0]  # k-ring x4, wave kernel, plain systolic
+    widths = {1: np.linspace(0.8, 1.2, 32), 4: np.linspace(1.3, 0.7, 32)}
+
+    kinds = [None] * len(cfgs)
+
+    def make(i):
+        hp = ka.HostProblem(ka.input_vars(**cfgs[i]), ctx)
+        if i in widths:  # zone widths along x vary: the tables' rows are not uniform
+            dx = hp.get("dx", 0)
+            hp.set("dx", 0, dx * widths[i])
+        kinds[i] = hp.kernels().set_tuning(v_max=tuning[i])
+        hp.init_psi()
+        return hp
+
+    ref_pops, ref_phi = [], []
+    for i in range(len(cfgs)):
+        hp = make(i)
+        ref_pops.append([hp.iterate(1) for _ in range(3)])
+        ref_phi.append([hp.get("phi", k) for k in range(hp.num_pr)])
+        hp.close()
+    hps = [make(i) for i in range(len(cfgs))]
+    assert kinds == [4, 4, 4, 4, 3, 0], kinds
+    b = ka.Batch(ctx, hps)
+    p1 = b.iterate(2)
+    p2 = b.iterate(1)
+    got = np.concatenate([p1, p2], axis=1)
+    for i in range(len(cfgs)):
+        assert np.array_equal(got[i], np.array(ref_pops[i])), "problem %d: %s vs %s" % (i, got[i], ref_pops[i])
+        for k in range(hps[i].num_pr):
+            assert np.array_equal(hps[i].get("phi", k), ref_phi[i][k])
+    b.close()
+    for hp in hps:
+        hp.close()
+
