@@ -502,3 +502,34 @@ def test_sweep_transpose_wave_kernel(ctx, layout, zones, zset, groups, quad, ker
     k.sweep_solver_adjoint(seeded=True)
     Aty = _get(hp, "rhs", hp.num_sdoms)
     _check_dot((y + fb, Hx + Bx), (Aty, x))
+
+
+@pytest.mark.parametrize("layout", ["DZG", "GZD", "ZDG", "ZGD"])
+@pytest.mark.parametrize("kernel,zones,zset,groups,quad", [((0, 0), (16, 8, 12), (2, 2, 1), 32, (4, 4)),
+                                                           ((3, 4), (32, 16, 32), (1, 1, 1), 64, (4, 8))])
+def test_sweep_transpose_varying_widths(ctx, layout, zones, zset, groups, quad, kernel):
+    """The seeded transpose identity on a mesh whose zone widths vary along
+    every axis (the systolic kernel's instance for whole table rows), plain
+    and with the faces between k-patches in LDS."""
+    hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=groups, zones=zones, quad=quad, legendre=1))
+    k = hp.kernels()
+    assert k.set_tuning(v_max=kernel[0]) == kernel[1]
+    rng = np.random.default_rng(49)
+    for ax, name in enumerate(("dx", "dy", "dz")):
+        for c in range(zset[ax]):
+            w = hp.get(name, c)
+            hp.set(name, c, w * (0.6 + 0.8 * rng.random(w.size)))
+    exits = [(s, PLANES[a]) for s in range(hp.num_sdoms) for a, e in enumerate(k.sdom_exits(s)) if e]
+    x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.sweep_solver(list(range(hp.num_sdoms)))
+    Hx = _get(hp, "psi", hp.num_sdoms)
+    Bx = [hp.get(name, s) for s, name in exits]
+    y = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    fb = []
+    for s, name in exits:
+        v = rng.standard_normal(hp.get(name, s).size)
+        hp.set(name, s, v)
+        fb.append(v)
+    k.sweep_solver_adjoint(seeded=True)
+    Aty = _get(hp, "rhs", hp.num_sdoms)
+    _check_dot((y + fb, Hx + Bx), (Aty, x))
