@@ -316,7 +316,7 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   const int ncoef = nd * (ni + nj + nk);
   const gmem<double> *src = gp(P.coef);  // A[nd][ni], B[nd][nj], C[nd][nk], then the uniform flag
   {
-    const bool fast = src[ncoef] != 0.0 && (!POP || P.vol_info[0] != 0.0);
+    const bool fast = src[ncoef] != 0.0 && (!POP || gp(P.vol_info)[0] != 0.0);
     if (fast != UNI) return;  // the whole workgroup: its subdomain is the other instance's
   }
   const int nt = blockDim.x;
@@ -409,7 +409,9 @@ __device__ __forceinline__ void cols_body(const ColArgs &P, const ColConst &K, d
   // zone volumes: one value for the whole subdomain (the UNI instance), else
   // one load per zone step, issued a step ahead and before that step's ring
   // loads (so its wait does not wait for them)
-  const double vol0 = POP ? P.vol_info[1] : 0.0;
+  // (a global-space read: a generic one would leave a flat access pending at
+  // the loop head, and every wait in the loop would then drain the ring)
+  const double vol0 = POP ? gp(P.vol_info)[1] : 0.0;
   double rv = vol0;
   if (POP && !UNI) rv = gvol[cc.vol8 / 8u];
 #pragma unroll
