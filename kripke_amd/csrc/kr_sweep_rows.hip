@@ -371,12 +371,16 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
   // issued before the ring refill so waiting for it leaves the ring in flight
   double rv[V];
 
-  RowCursor lc, cc;
-  cursor_init(lc, P, K, -kk, 8u * (uint32_t)(k * nj * ni + mb), 8u * (uint32_t)(k * nj),
+  // Each ring slot holds the cursor of the row it is loaded for, which is
+  // the compute's cursor when the slot is consumed; a refill steps the
+  // previous slot's cursor (the row before) into its own slot, so one cursor
+  // update per step serves both the loads and the compute.
+  RowCursor rc[RING];
+  cursor_init(rc[0], P, K, -kk, 8u * (uint32_t)(k * nj * ni + mb), 8u * (uint32_t)(k * nj),
               8u * (uint32_t)(k * ni + mb), mb8);
-  cc = lc;
 
   auto refill = [&](int r) {
+    const RowCursor &lc = rc[r];
     load_v<V, NT>(rq[r], Rrhs, lc.zo8);
     load_v<V, 0>(rs[r], Rsig, lc.so8);
     // faces: the k-face only for row group 0, the j-face only on a patch's first row,
@@ -388,21 +392,26 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
       load_v<1, 0>(t, Rin[0], l == 0 ? lc.xo8 : NOWHERE);
       rx[r] = t[0];
     }
-    cursor_step(lc, P, K);
   };
 
   // volumes: one value for the whole subdomain, or one L2 load per row
-  auto vol_row = [&]() {
+  auto vol_row = [&](uint32_t vo8) {
     if (P.vol_uniform) {
 #pragma unroll
       for (int e = 0; e < V; ++e) rv[e] = P.vol0;
     } else {
-      load_v<V, 0>(rv, Rvol, cc.vo8);
+      load_v<V, 0>(rv, Rvol, vo8);
     }
   };
-  if (POP) vol_row();
+  if (POP) vol_row(rc[0].vo8);
 #pragma unroll
-  for (int r = 0; r < RING; ++r) refill(r);
+  for (int r = 0; r < RING; ++r) {
+    if (r > 0) {
+      rc[r] = rc[r - 1];
+      cursor_step(rc[r], P, K);
+    }
+    refill(r);
+  }
 
   double fy[V];
 #pragma unroll
@@ -415,6 +424,7 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
     for (int r = 0; r < RING; ++r) {
       const int t = t0 + r;
       const int cur = t & 1, prev = cur ^ 1;
+      const RowCursor &cc = rc[r];
       const bool act = cc.pos >= 0 && cc.pos < K.rows;
       const double bj = co[cc.bco + ni + (P.jd > 0 ? cc.jj : nj - 1 - cc.jj)];
       const int kc = P.kd > 0 ? cc.kp * PK + kk : nk - 1 - cc.kp * PK - kk;  // this row group's k-row
@@ -508,8 +518,13 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
       }
 #pragma unroll
       for (int m = 0; m < V; ++m) fzb[(cur * nt + tid) * V + m] = fzo[memo<SX, V>(m)];
-      cursor_step(cc, P, K);
-      if (POP) vol_row();
+      // the next step's row volumes (its slot was filled a step earlier)
+      if (POP) vol_row(rc[(r + 1) % RING].vo8);
+      {
+        RowCursor nc = rc[(r + RING - 1) % RING];
+        cursor_step(nc, P, K);
+        rc[r] = nc;
+      }
       refill(r);
       __syncthreads();
     }
