@@ -297,11 +297,14 @@ int kr_diff_backward(kr_diff *diff, int32_t first, int32_t num, const double *co
 int kr_problem_autotune(kr_problem *prob, int32_t *choice);
 /* No counterpart in the reference.  Installs one point of the autotuner's
  * search space for the pipelined sweep instead of timing them all:
- * v_max caps the row-scan kernel's zones per lane (zone-fastest nestings,
- * 0 = no cap) or, in the other nestings, selects the kernel (0 = systolic
- * pencils, 2 = wave-uniform planes, 3 = systolic pencils with the patches
- * in k-fastest order and the faces between k-patches in LDS, each where the
- * shape fits it); cap_div shrinks the systolic patch (both forms) or the
+ * v_max sets the row-scan kernel's zones per lane in the zone-fastest
+ * nestings (0, 2 or 3 = two, the default; 1 = one; 4 = four where the rows,
+ * of at most 64 zones, divide by four) or, in the other nestings, selects
+ * the kernel (0 or 4 = systolic pencils, 2 = wave-uniform planes, 3 =
+ * systolic pencils with the patches in k-fastest order and the faces
+ * between k-patches in LDS, each where the shape fits it); a shape a choice
+ * does not fit falls back to fewer zones per lane or to the plain systolic
+ * kernel.  cap_div shrinks the systolic patch (both forms) or the
  * wave kernel's waves per workgroup (16 / cap_div);
  * r_scale_pct scales the pair groups per work unit.  kr_problem_sweep_kind
  * reports the kernel installed: 0 systolic, 1 row scan, 3 wave-uniform,

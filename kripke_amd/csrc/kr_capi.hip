@@ -1063,7 +1063,7 @@ static int install_choice(kr_problem *p, const TuneChoice &c) {
 }
 
 int kr_problem_set_tuning(kr_problem *p, int32_t v_max, int32_t cap_div, int32_t r_scale_pct) {
-  if (!p || v_max < 0 || v_max > 3 || cap_div < 1 || cap_div > 16 || r_scale_pct < 1 || r_scale_pct > 10000)
+  if (!p || v_max < 0 || v_max > 4 || cap_div < 1 || cap_div > 16 || r_scale_pct < 1 || r_scale_pct > 10000)
     return fail(KR_ERR_INVALID, "kr_problem_set_tuning: bad argument");
   TuneChoice c;
   c.v_max = v_max;
@@ -1090,7 +1090,8 @@ int kr_problem_autotune(kr_problem *p, int32_t *choice) {
     const bool zfast = p->nesting == KR_DGZ || p->nesting == KR_GDZ;
     for (int pct : {50, 100, 200}) {
       if (zfast) {
-        for (int v : {0, 1}) {
+        // two zones per lane (the default), one, or four
+        for (int v : {0, 1, 4}) {
           TuneChoice c;
           c.v_max = v;
           c.r_scale_pct = pct;

@@ -636,27 +636,30 @@ static int largest_divisor_le(int n, int cap) {
 
 // Row-scan shape for zone-fastest nestings: V zones per lane (V > 1 needs
 // 16-byte aligned rows), W >= ni/V lanes per row group, one row group per
-// k-row.  Returns false when the subdomain does not fit one workgroup.
+// k-row.  Two zones per lane by default; four (rows of up to 64 zones, at
+// most 512 threads: that kernel's launch bound) only when v_max asks for
+// them.  Returns false when the subdomain does not fit one workgroup.
 static bool row_shape(SweepShape &sh, int32_t nd, int32_t ng, int32_t ni, int32_t nj, int32_t nk, bool aligned16,
                       int v_max, size_t lds_limit) {
-  int want = ni % 2 == 0 ? 2 : 1;
-  if (v_max > 0) want = std::min(want, v_max);
+  int want = ni % 4 == 0 ? 4 : ni % 2 == 0 ? 2 : 1;
+  want = v_max > 0 ? std::min(want, v_max) : std::min(want, 2);
   const int64_t nz = (int64_t)ni * nj * nk;
   const int64_t maxface = std::max({(int64_t)nj * nk, (int64_t)ni * nk, (int64_t)ni * nj});
   // every stream is a buffer resource based at the unit's first item and
   // addressed with 32-bit byte offsets below 2 GiB (units hold <= 64 items)
   if (64 * nz * 8 >= (1LL << 31) || 64 * maxface * 8 >= (1LL << 31) || (int64_t)ng * nz * 8 >= (1LL << 31))
     return false;
-  for (int V = 2; V >= 1; V /= 2) {
+  for (int V = 4; V >= 1; V /= 2) {
     if (V > want || ni % V != 0 || (V > 1 && !aligned16)) continue;
     int W = 4;
     while (W * V < ni) W *= 2;
-    if (W > 64) continue;
+    if (W > (V == 4 ? 16 : 64)) continue;
+    const int tmax = V == 4 ? 512 : 1024;
     // k-rows per block: all of them, or a divisor of nk; between k-patches the
     // k-face goes through the own plane and must be written (by the patch's
     // top row group, PK-1 steps into a row) before the next patch prefetches
     // it (4 steps ahead): nj >= PK + 4
-    for (int PK = largest_divisor_le(nk, 1024 / W); PK >= 1; PK = largest_divisor_le(nk, PK - 1)) {
+    for (int PK = largest_divisor_le(nk, tmax / W); PK >= 1; PK = largest_divisor_le(nk, PK - 1)) {
       const size_t lds = (2 * (size_t)W * PK * V + (size_t)nd * (ni + nj + nk + 1)) * sizeof(double);
       if ((PK == nk || nj >= PK + 4) && lds <= lds_limit) {
         sh.kind = 1;

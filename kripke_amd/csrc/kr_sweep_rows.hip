@@ -49,16 +49,20 @@ __device__ __forceinline__ Rsrc rsrc(const void *p, uint32_t bytes) {
 
 constexpr int NT = 2;  // cache policy: non-temporal (streamed once)
 
-// V consecutive doubles at byte offset off (16-byte aligned when V = 2).
+// V consecutive doubles at byte offset off (16-byte aligned when V > 1).
 template <int V, int AUX>
 __device__ __forceinline__ void load_v(double (&t)[V], Rsrc r, uint32_t off) {
   if constexpr (V == 1) {
     t[0] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
   } else {
-    static_assert(V == 2, "V is 1 or 2");
-    const d2 w = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
-    t[0] = w.x;
-    t[1] = w.y;
+    static_assert(V == 2 || V == 4, "V is 1, 2 or 4");
+#pragma unroll
+    for (int h = 0; h < V / 2; ++h) {
+      // (a NOWHERE offset stays out of range with the 16 bytes added)
+      const d2 w = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, off + 16u * h, 0, AUX));
+      t[2 * h] = w.x;
+      t[2 * h + 1] = w.y;
+    }
   }
 }
 
@@ -67,10 +71,13 @@ __device__ __forceinline__ void store_v(Rsrc r, uint32_t off, const double (&t)[
   if constexpr (V == 1) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, t[0]), r, off, 0, AUX);
   } else {
-    d2 w;
-    w.x = t[0];
-    w.y = t[1];
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, w), r, off, 0, AUX);
+#pragma unroll
+    for (int h = 0; h < V / 2; ++h) {
+      d2 w;
+      w.x = t[2 * h];
+      w.y = t[2 * h + 1];
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, w), r, off + 16u * h, 0, AUX);
+    }
   }
 }
 
@@ -548,7 +555,7 @@ __device__ __forceinline__ void rows_body(const RowArgs &P, int PK, double *lds)
 // ADJ: the transposed sweep, psi-shaped rhsbar = H^-T psibar: it reads the
 // psi field, writes the rhs field and runs every axis in reverse.
 template <int W, int V, bool POP, bool ADJ>
-__global__ __launch_bounds__(1024) void k_sweep_rows(const DevSdom *__restrict__ sd,
+__global__ __launch_bounds__(V == 4 ? 512 : 1024) void k_sweep_rows(const DevSdom *__restrict__ sd,
                                                      const SweepPlanes *__restrict__ planes,
                                                      const SweepUnit *__restrict__ units, SweepShape sh,
                                                      double *__restrict__ pop) {
@@ -616,6 +623,7 @@ int launch_sweep_rows(hipStream_t st, const DevSdom *sd, const SweepPlanes *plan
     break;
     X(4, 1) X(8, 1) X(16, 1) X(32, 1) X(64, 1)
     X(4, 2) X(8, 2) X(16, 2) X(32, 2) X(64, 2)
+    X(4, 4) X(8, 4) X(16, 4)
 #undef X
     default: set_error("row sweep: unsupported row-group shape"); return KR_ERR_UNSUPPORTED;
   }

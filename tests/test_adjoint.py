@@ -504,6 +504,32 @@ def test_sweep_transpose_wave_kernel(ctx, layout, zones, zset, groups, quad, ker
     _check_dot((y + fb, Hx + Bx), (Aty, x))
 
 
+@pytest.mark.parametrize("layout", ["DGZ", "GDZ"])
+@pytest.mark.parametrize("zones,zset", [((32, 8, 12), (1, 1, 1)), ((16, 12, 20), (2, 2, 2))])
+def test_sweep_transpose_four_zones_per_lane(ctx, layout, zones, zset):
+    """The seeded transpose identity with the row-scan kernel at four zones
+    per lane installed (kr_problem_set_tuning v_max = 4): forward and adjoint
+    sweeps both run it."""
+    hp = product_for(ctx, **kwargs_for(layout, zset=zset, groups=16, zones=zones, quad=(4, 4), legendre=1))
+    k = hp.kernels()
+    assert k.set_tuning(v_max=4) == 1
+    rng = np.random.default_rng(47)
+    exits = [(s, PLANES[a]) for s in range(hp.num_sdoms) for a, e in enumerate(k.sdom_exits(s)) if e]
+    x = _rand_fields(hp, "rhs", hp.num_sdoms, rng)
+    k.sweep_solver(list(range(hp.num_sdoms)))
+    Hx = _get(hp, "psi", hp.num_sdoms)
+    Bx = [hp.get(name, s) for s, name in exits]
+    y = _rand_fields(hp, "psi", hp.num_sdoms, rng)
+    fb = []
+    for s, name in exits:
+        v = rng.standard_normal(hp.get(name, s).size)
+        hp.set(name, s, v)
+        fb.append(v)
+    k.sweep_solver_adjoint(seeded=True)
+    Aty = _get(hp, "rhs", hp.num_sdoms)
+    _check_dot((y + fb, Hx + Bx), (Aty, x))
+
+
 @pytest.mark.parametrize("layout", ["DZG", "GZD", "ZDG", "ZGD"])
 @pytest.mark.parametrize("kernel,zones,zset,groups,quad", [((0, 0), (16, 8, 12), (2, 2, 1), 32, (4, 4)),
                                                            ((3, 4), (32, 16, 32), (1, 1, 1), 64, (4, 8))])

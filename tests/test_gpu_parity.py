@@ -698,3 +698,49 @@ def test_sweep_nonuniform_mesh(ctx, layout, kernel):
     _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=layout not in ZFAST)
     _, comp = orf.population()
     assert abs(pop - comp) <= TOL_REF * abs(comp)
+
+
+# four zones per lane in the row-scan kernel (tuning point v_max = 4): rows of
+# 32 zones (8 lanes), 28 (7 of 8 lanes live), 64 (16 lanes) and 8 (2 of 4
+# lanes, eight zone sets)
+ROWS4_SHAPES = [dict(zones=(32, 8, 12), zset=(1, 1, 1)), dict(zones=(28, 8, 8), zset=(1, 1, 1)),
+                dict(zones=(64, 4, 8), zset=(1, 1, 1)), dict(zones=(16, 12, 20), zset=(2, 2, 2))]
+
+
+@pytest.mark.parametrize("layout", ZFAST)
+@pytest.mark.parametrize("shape", range(len(ROWS4_SHAPES)))
+def test_row_scan_four_zones_per_lane(ctx, layout, shape):
+    """Row-scan sweep with four zones per lane: the pipelined sweep with the
+    fused population and the plain one against both oracles (normwise: the
+    scan re-associates the row recurrence), psi different in its last bits
+    from the two-zone kernel's (so the four-zone kernel ran), and three
+    source iterations against the reference-arithmetic oracle."""
+    kw = kwargs_for(layout, **ROWS4_SHAPES[shape])
+    hp, of, orf = _trio(ctx, **kw)
+    two = product_for(ctx, **kw)
+    assert hp.kernels().set_tuning(v_max=4) == 1
+    rng = np.random.default_rng(90 + shape)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    for t in range(hp.num_sdoms):
+        two.set("rhs", t, hp.get("rhs", t))
+    pop = hp.kernels().sweep_solver_population(list(range(hp.num_sdoms)))
+    two.kernels().sweep_solver_population(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=False)
+    _, comp = orf.population()
+    assert abs(pop - comp) <= TOL_REF * abs(comp)
+    assert any(not np.array_equal(hp.get("psi", t), two.get("psi", t)) for t in range(hp.num_sdoms))
+    hp.kernels().sweep_solver(list(range(hp.num_sdoms)))
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=False)
+    two.close()
+    hp.close()
+    b, ob = product_for(ctx, **kw), oracle_for("ref", **kw)
+    assert b.kernels().set_tuning(v_max=4) == 1
+    b.init_psi()
+    ob.init_psi()
+    for it in range(3):
+        p = b.iterate(1)
+        _, c = ob.iterate()
+        assert abs(p - c) <= TOL_REF * abs(c), (it, p, c)
+    b.close()
