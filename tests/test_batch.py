@@ -270,3 +270,41 @@ def test_batch_systolic_tuning_points(ctx):
     for hp in hps:
         hp.close()
 
+
+
+def test_batch_row_scan_four_zones_per_lane(ctx):
+    """Zone-fastest problems with the row-scan kernel at four zones per lane
+    installed (rows of 32 and 16 zones, one mesh with varying zone widths)
+    beside one at the default two, in one batch: populations and phi equal
+    each problem's own runs bit for bit."""
+    base = dict(zset=(1, 1, 1), groups=32, quad=(4, 8), legendre=2, dset=8, gset=1, geometry=1)
+    cfgs = [dict(base, zones=(32, 16, 12), layout="DGZ", seed=800), dict(base, zones=(16, 8, 8), layout="GDZ", seed=801),
+            dict(base, zones=(32, 8, 16), layout="GDZ", seed=802), dict(base, zones=(32, 16, 12), layout="DGZ", seed=803)]
+    tuning = [4, 4, 4, 0]
+    widths = {2: np.linspace(0.8, 1.2, 32)}
+
+    def make(i):
+        hp = ka.HostProblem(ka.input_vars(**cfgs[i]), ctx)
+        if i in widths:
+            dx = hp.get("dx", 0)
+            hp.set("dx", 0, dx * widths[i])
+        assert hp.kernels().set_tuning(v_max=tuning[i]) == 1
+        hp.init_psi()
+        return hp
+
+    ref_pops, ref_phi = [], []
+    for i in range(len(cfgs)):
+        hp = make(i)
+        ref_pops.append([hp.iterate(1) for _ in range(3)])
+        ref_phi.append([hp.get("phi", k) for k in range(hp.num_pr)])
+        hp.close()
+    hps = [make(i) for i in range(len(cfgs))]
+    b = ka.Batch(ctx, hps)
+    got = np.concatenate([b.iterate(2), b.iterate(1)], axis=1)
+    for i in range(len(cfgs)):
+        assert np.array_equal(got[i], np.array(ref_pops[i])), "problem %d: %s vs %s" % (i, got[i], ref_pops[i])
+        for k in range(hps[i].num_pr):
+            assert np.array_equal(hps[i].get("phi", k), ref_phi[i][k])
+    b.close()
+    for hp in hps:
+        hp.close()
