@@ -744,3 +744,24 @@ def test_row_scan_four_zones_per_lane(ctx, layout, shape):
         _, c = ob.iterate()
         assert abs(p - c) <= TOL_REF * abs(c), (it, p, c)
     b.close()
+
+
+@pytest.mark.parametrize("layout", ZFAST)
+def test_row_scan_four_zones_nonuniform_mesh(ctx, layout):
+    """The row-scan sweep at four zones per lane on a mesh whose zone widths
+    and volumes vary (each lane's four a values, the per-row volume loads):
+    psi normwise against both oracles, the fused population against the
+    oracle's compensated sum."""
+    shape = dict(zones=(32, 8, 12), zset=(2, 2, 1), groups=32, quad=(4, 4))
+    kw = kwargs_for(layout, **shape)
+    hp, of, orf = _trio(ctx, **kw)
+    assert hp.kernels().set_tuning(v_max=4) == 1
+    rng = np.random.default_rng(71)
+    _vary_mesh(hp, [of, orf], shape["zset"], rng)
+    seeded_fill(hp, [of, orf], "rhs", hp.num_sdoms, rng, offset=1.0)
+    pop = hp.kernels().sweep_solver_population(list(range(hp.num_sdoms)))
+    of.sweep_solver()
+    orf.sweep_solver()
+    _assert_chunks(hp, of, orf, "psi", hp.num_sdoms, tol=1e-13, exact=False)
+    _, comp = orf.population()
+    assert abs(pop - comp) <= TOL_REF * abs(comp)
